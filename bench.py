@@ -1,0 +1,188 @@
+"""MGMC sampling benchmark on one MI355X.
+
+Workload: the 128^3 3D shifted-Laplace posterior (kappa^2 = 10, Dirichlet
+boundaries) with 32 top-hat ball measurements (radius 0.05, centres
+U[0.1, 0.9]^3, variance 1e-4, y ~ N(0, 1)), MGMC V-cycle with SSOR 1+1 Gibbs
+smoothing (omega = 1), 5 levels (coarsest 7^3 = 343 unknowns, Cholesky).
+A step is one MGMC cycle over the whole batch of chains.
+
+value = IACT-adjusted independent samples per second of the batched FP32 form
+(64 chains): K * steps / (t_wall * tau_int), tau_int the largest Sokal
+estimate over the probe vertices. The plain single-chain FP64 form is
+measured in the same run and reported beside it, with the ratio.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IACT-adjusted indep. samples/s at 128^3 posterior: batched FP32 vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def problem(n, m, seed):
+    import multigridmc_amd as mg
+    lat = mg.Lattice(n, n, n)
+    centres, y, var = mg.seeded_ball_measurements(lat, m, 0.05, 1e-4, seed)
+    op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
+    # f = Q mean_exact = B Sigma^{-1} y for a zero prior mean (src/measurements/sampling_time.cc:19-28)
+    f = op.get_B() @ (y / op.get_Sigma())
+    return op, f, centres, y, var
+
+
+def tau_max(obs):
+    import multigridmc_amd as mg
+    taus = [mg.tau_int_chains(obs[:, :, p])[0] for p in range(obs.shape[2])]
+    return float(max(taus)), float(np.median(taus))
+
+
+def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed):
+    import multigridmc_amd as mg
+    s = mg.MultigridMCSampler(op, params, nchains=nchains, precision=precision, seed=seed)
+    s.fix_rhs(f)
+    s.set_state(np.zeros((nchains, op.lattice.Nvertex)))
+    s.set_probes(probes)
+    s.run(warmup, record=False)
+    t0 = time.perf_counter()
+    obs = s.run(steps)  # synchronises the handle's stream before returning
+    t = time.perf_counter() - t0
+    return s, obs, t
+
+
+def cpu_baseline(n, m, seed, budget_s, nthreads):
+    """The oracle (FP64 restatement of the reference: lexicographic SSOR Gibbs
+    sweeps with dense SMW matrices, CLCG noise) on the host cores."""
+    from oracle import pyoracle as po
+    import multigridmc_amd as mg
+    lat = mg.Lattice(n, n, n)
+    centres, y, var = mg.seeded_ball_measurements(lat, m, 0.05, 1e-4, seed)
+    ora = po.Operator.fd(3, (n, n, n), 10.0)
+    ora.add_measurements(centres, 0.05, var)
+    omg = po.Multigrid(ora, 5, ordering=po.LEX)
+    f = ora.csr(1) @ (y / var)
+    N = ora.N
+    # single thread: ms per sample (src/measurements/sampling_time.cc:36-44)
+    x, _, t1 = omg.chains(f, np.zeros((1, N)), 1, nthreads=1)
+    ns1 = max(1, min(20, int(0.4 * budget_s / max(t1, 1e-3))))
+    x, _, t1 = omg.chains(f, x, ns1, nthreads=1)
+    ms_single = 1e3 * t1 / ns1
+    # all cores: one chain per thread (src/measurements/convergence.cc:29-63)
+    ns = max(1, min(10, int(0.5 * budget_s / max(ms_single * 1e-3 * 1.5, 1e-3))))
+    _, _, tm = omg.chains(f, np.zeros((nthreads, N)), ns, nthreads=nthreads)
+    rate = nthreads * ns / tm
+    return {
+        "value": rate,
+        "unit": "chain-samples/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"oracle MGMC (lexicographic SSOR 1+1, dense SMW, FP64) at {n}^3 with {m} measurements: "
+                  f"{nthreads} chains x {ns} cycles on {nthreads} threads; single thread {ns1} cycles",
+        "ms_per_sample_1thread": ms_single,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--chains", type=int, default=64)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--m", type=int, default=32)
+    ap.add_argument("--fp64-steps", type=int, default=2000)
+    ap.add_argument("--probes", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (chains would shard across "
+              f"GPUs with no per-step communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import multigridmc_amd as mg
+
+    n, K = args.n, args.chains
+    op, f, centres, y, var = problem(n, args.m, args.seed)
+    N = op.lattice.Nvertex
+    rng = np.random.default_rng(args.seed + 1)
+    probes = np.sort(rng.choice(N, size=args.probes, replace=False))
+    params = mg.MultigridParameters(nlevel=5, smoother="SSOR", npresmooth=1, npostsmooth=1, omega=1.0, cycle=1,
+                                    coarse_solver="Cholesky")
+
+    # batched FP32 (the metric's form)
+    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1)
+    tau32, tau32_med = tau_max(obs32)
+    ess32 = K * args.steps / (t32 * tau32)
+
+    # dominant kernel: level-0 colour-phase sweep, timed with HIP events on the handle's stream
+    s32.profile(1)
+    s32.run(10, record=False)
+    ms_sweep, nlaunch = s32.profile_result()
+    s32.profile(0)
+    w = 4
+    bytes_per_launch = 2 * w * N * K  # read x (all), read f + write x (one colour)
+    achieved = bytes_per_launch / (ms_sweep * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_l0_fp32_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    # plain single-chain FP64
+    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2)
+    tau64, tau64_med = tau_max(obs64)
+    ess64 = args.fp64_steps / (t64 * tau64)
+
+    out = {
+        "metric": METRIC,
+        "value": ess32,
+        "unit": "indep. samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n}^3 shifted-Laplace posterior, kappa^2=10, {args.m} ball measurements r=0.05 var=1e-4, "
+                        f"MGMC V(1,1) SSOR omega=1, 5 levels, Cholesky 343 coarse, {K} chains FP32",
+            "chains": K,
+            "lattice": [n, n, n],
+            "measurements": args.m,
+            "nlevel": 5,
+            "probes": args.probes,
+        },
+        "tau_int": {"fp32_batched_max": tau32, "fp32_batched_median": tau32_med, "fp64_single_max": tau64,
+                    "fp64_single_median": tau64_med, "estimator": "Sokal automatic window c=5"},
+        "fp64_single_chain": {"value": ess64, "unit": "indep. samples/s", "steps": args.fp64_steps,
+                              "ms_per_step": 1e3 * t64 / args.fp64_steps},
+        "speedup_vs_fp64_single": ess32 / ess64,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sweep<float,3,star,philox> (level-0 red-black colour phase)",
+                     "avg_launch_ms": ms_sweep, "launches_timed": nlaunch,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, args.m, args.seed, args.cpu_budget, nthreads)
+        except Exception as e:  # the GPU line must still print
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

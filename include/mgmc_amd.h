@@ -1,0 +1,169 @@
+/* mgmc_amd.h - C ABI of the MI355X (gfx950) multigrid Monte Carlo library.
+ *
+ * The library implements the MultigridMC hot path behind plain C entry
+ * points: the multigrid Monte Carlo (MGMC) sampler and its zero-noise twin,
+ * the multigrid preconditioner with the Richardson loop solver. Host code
+ * above this ABI (the C++ adapters in multigridmc_amd/csrc/adapters and the
+ * Python package multigridmc_amd) mirrors the reference's class interfaces.
+ *
+ * Conventions
+ *  - Every vector handed across the ABI is FP64 in the reference's vertex
+ *    order: interior vertices only, x fastest (src/lattice/lattice3d.hh:132-141).
+ *    Batched arguments are [chain][vertex] rows.
+ *  - Pointers are host pointers unless the name ends in _dev.
+ *  - Every function returns 0 on success and a nonzero mgmc_status otherwise;
+ *    mgmc_last_error() gives a message. Nothing in the library calls exit().
+ *  - One handle per host thread; work is issued on the handle's HIP stream and
+ *    host-pointer functions synchronise that stream before returning.
+ */
+#ifndef MGMC_AMD_H
+#define MGMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mgmc_handle mgmc_handle;
+
+enum mgmc_status {
+  MGMC_OK = 0,
+  MGMC_ERR_ARG = 1,         /* invalid argument (sizes, levels, parameters)   */
+  MGMC_ERR_UNSUPPORTED = 2, /* operator the kernels do not cover              */
+  MGMC_ERR_NOT_SPD = 3,     /* coarse matrix not positive definite            */
+  MGMC_ERR_HIP = 4,         /* HIP runtime error                              */
+  MGMC_ERR_NOCONV = 5,      /* solver reached maxiter                         */
+  MGMC_ERR_STATE = 6        /* call out of order (e.g. no probes set)         */
+};
+
+enum mgmc_smoother { MGMC_SOR = 0, MGMC_SSOR = 1 };
+enum mgmc_coarse { MGMC_COARSE_CHOLESKY = 0, MGMC_COARSE_SSOR = 1 };
+enum mgmc_precision { MGMC_FP64 = 0, MGMC_FP32 = 1 };
+
+/* Posterior precision Q = A + B Sigma^{-1} B^T on the fine lattice
+ * (src/linear_operator/linear_operator.hh:28-213).
+ * A: if A_rowptr is NULL the library assembles ShiftedLaplaceFDOperator with
+ * constant kappa^2 (src/linear_operator/shiftedlaplace_fd_operator.cc:9-57);
+ * otherwise A is taken as given (CSR, symmetric), and must be a constant-
+ * coefficient nearest-neighbour stencil with Dirichlet elimination.
+ * B: N x m CSR (column k = measurement k), Sigma: m variances (already
+ * multiplied by variance_scaling, src/linear_operator/measured_operator.cc:19). */
+typedef struct {
+  int dim;   /* 2 or 3 */
+  int n[3];  /* cells per direction (nx, ny, nz); unknowns = prod(n_d - 1) */
+  double kappa_sq;
+  const int64_t* A_rowptr;
+  const int64_t* A_col;
+  const double* A_val;
+  int m;
+  const int64_t* B_rowptr;
+  const int64_t* B_col;
+  const double* B_val;
+  const double* sigma;
+} mgmc_problem;
+
+/* MultigridParameters (src/auxilliary/parameters.hh:150-179) plus the
+ * MI355X-side choices (precision, number of chains, RNG seed, stream). */
+typedef struct {
+  int nlevel;
+  int smoother;       /* mgmc_smoother */
+  int npresmooth;
+  int npostsmooth;
+  int ncoarsesmooth;  /* SSOR coarse sampler only */
+  double omega;
+  int cycle;          /* 1 = V, 2 = W */
+  double coarse_scaling;
+  int coarse_solver;  /* mgmc_coarse (sampler); the preconditioner is always Cholesky */
+  int precision;      /* mgmc_precision of the cycle (sweeps, transfers) */
+  int nchains;        /* K independent chains batched in every launch */
+  uint64_t seed;      /* Philox4x32-10 key */
+  void* stream;       /* hipStream_t, NULL = a stream owned by the handle */
+} mgmc_params;
+
+/* Builds the hierarchy: replaces the constructors of MultigridMCSampler
+ * (src/sampler/multigridmc_sampler.cc:8-106) and MultigridPreconditioner
+ * (src/preconditioner/multigrid_preconditioner.cc:8-71). */
+int mgmc_create(const mgmc_problem* problem, const mgmc_params* params, mgmc_handle** out);
+int mgmc_destroy(mgmc_handle* h);
+const char* mgmc_last_error(void);
+
+/* Unknowns on `level` (Lattice::Nvertex, src/lattice/lattice.hh:126-131). */
+int mgmc_ndof(const mgmc_handle* h, int level, int64_t* n);
+
+/* Sampler::apply(f, x) for all chains (src/sampler/sampler.hh:48-49,
+ * src/sampler/multigridmc_sampler.cc:139-145): one MGMC cycle per chain.
+ * f: N values shared by all chains (f_per_chain = 0) or nchains x N.
+ * x: nchains x N, chain state on entry, new sample on exit. */
+int mgmc_sample(mgmc_handle* h, const double* f, int f_per_chain, double* x);
+
+/* Resident sampling (the loop of src/measurements/sampling_time.cc:30-41):
+ * the fine-level right-hand side and chain states stay in device memory.
+ * mgmc_set_rhs replaces Sampler::fix_rhs (src/sampler/sampler.hh:64). */
+int mgmc_set_rhs(mgmc_handle* h, const double* f, int f_per_chain);
+int mgmc_set_state(mgmc_handle* h, const double* x);
+int mgmc_get_state(mgmc_handle* h, double* x);
+/* Observables: values of x at `nprobe` fine vertices (reference order). */
+int mgmc_set_probes(mgmc_handle* h, int nprobe, const int64_t* vertices);
+/* Runs nsamples cycles on the device; if obs != NULL writes
+ * obs[s][chain][probe] (host) after each cycle. use_graph = 1 replays a
+ * captured hipGraph of the cycle. */
+int mgmc_run(mgmc_handle* h, int nsamples, double* obs, int use_graph);
+/* Same, with the observable buffer in device memory (nsamples x nchains x nprobe). */
+int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph);
+/* Reset the sample counter that, with the seed, keys the Philox noise. */
+int mgmc_set_counter(mgmc_handle* h, uint64_t sample_counter);
+
+/* MultigridPreconditioner::apply (src/preconditioner/multigrid_preconditioner.cc:104-109):
+ * one cycle from a zero initial guess, b, x: nchains x N. */
+int mg_precondition(mgmc_handle* h, const double* b, double* x);
+
+/* LoopSolver::apply (src/solver/loop_solver.cc:9-54): x = 0; repeat
+ * r = Q x - b (FP64), stop if ||r||/||r0|| < rtol and ||r|| < atol, else
+ * x -= MG(r). Chain 0 only. hist (maxiter entries, may be NULL) receives
+ * ||r_k||; *niter the converged iteration or -1. The cycle runs in the
+ * handle's precision, residuals and the update in FP64 (iterative refinement). */
+int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
+             int* niter);
+
+/* LinearOperator::apply (src/linear_operator/linear_operator.hh:78-88) on
+ * level 0 in FP64, all chains: y = Q x. */
+int mgmc_apply_operator(mgmc_handle* h, const double* x, double* y);
+
+/* ---- component entry points (test hooks; same kernels as the cycle) ---- */
+
+/* One SOR sweep on `level` in the handle's precision, all chains:
+ * forward = 1 / backward = 0; stochastic = 1 adds the Gibbs noise of
+ * SORSampler::apply (src/sampler/sor_sampler.cc:37-59), 0 is the smoother
+ * SORSmoother::apply with nsmooth = 1 (src/smoother/sor_smoother.cc:41-53).
+ * xi (nchains x N_level) and xi_lr (nchains x m) inject the normals; when
+ * NULL they come from the Philox stream. b, x: nchains x N_level. */
+int mgmc_sweep(mgmc_handle* h, int level, int forward, int stochastic, const double* b, double* x,
+               const double* xi, const double* xi_lr);
+/* fc = R (f - Q x) on level -> level+1 (multigridmc_sampler.cc:125-128). */
+int mgmc_residual_restrict(mgmc_handle* h, int level, const double* f, const double* x, double* fc);
+/* x += alpha P xc (src/intergrid/intergrid_operator.hh:106-120). */
+int mgmc_prolongate_add(mgmc_handle* h, int level, double alpha, const double* xc, double* x);
+/* Coarsest level: x = L^{-T}(xi + L^{-1} f) (src/sampler/cholesky_sampler.hh:52-66)
+ * with xi injected (nchains x Nc) or from Philox; xi_mode = 0: deterministic
+ * x = Q_c^{-1} f (src/solver/cholesky_solver.cc:30-40), 1: Philox, 2: injected. */
+int mgmc_coarse(mgmc_handle* h, const double* f, double* x, int xi_mode, const double* xi);
+/* Philox4x32-10 normals exactly as the sweeps draw them: for every chain and
+ * fine vertex (reference order) the normal of sweep `tag` at the current
+ * sample counter. out: nchains x N. */
+int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out);
+/* Export the level's stencil as used by the kernels (27 or 9 entries, x fastest,
+ * offsets -1..1 per direction) and the colour count (2 = red-black). */
+int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);
+
+/* Profiling aid: with enable = 1 every level-0 sweep colour-phase launch is
+ * bracketed by HIP events on the handle's stream (runs then launch eagerly).
+ * mgmc_profile_result returns the mean duration of the launches recorded
+ * since the last call and their number. No reference counterpart. */
+int mgmc_profile(mgmc_handle* h, int enable);
+int mgmc_profile_result(mgmc_handle* h, double* avg_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGMC_AMD_H */

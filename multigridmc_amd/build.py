@@ -1,0 +1,33 @@
+"""Builds libmgmc_amd.so in-tree for gfx950 (hipcc) from multigridmc_amd/csrc."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmgmc_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# Packed FP32 (v_pk_*_f32) code generation is disabled for every HIP file.
+HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
+
+
+def build(verbose=True):
+    obj = os.path.join(CSRC, "setup.o")
+    hobj = os.path.join(CSRC, "mgmc_lib.o")
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "setup.cc"), "-o", obj],
+        [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mgmc_lib.hip"), "-o", hobj],
+        [HIPCC, "--offload-arch=gfx950", "-shared", hobj, obj, "-o", LIB + ".tmp"],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        subprocess.check_call(c)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

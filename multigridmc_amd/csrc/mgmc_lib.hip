@@ -1,0 +1,964 @@
+// Cycle engine and C ABI (include/mgmc_amd.h) of the MGMC library.
+//
+// The host drives the recursion of MultigridMCSampler::sample
+// (src/sampler/multigridmc_sampler.cc:109-136) and
+// MultigridPreconditioner::solve (src/preconditioner/multigrid_preconditioner.cc:74-101)
+// by issuing kernels on one HIP stream; every launch covers all K chains.
+// A whole cycle (plus the observable and counter update) can be captured in
+// a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mgmc_amd.h"
+#include "kernels.hip"
+#include "setup.h"
+
+namespace mgmc {
+
+static thread_local std::string g_error;
+
+#define HIP_CHECK(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) throw Error(MGMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+static unsigned nblocks(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_failed(hipError_t e) { throw Error(MGMC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e)); }
+
+template <typename X>
+struct DevBuf {
+  X* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    n = count;
+    if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(X)));
+  }
+  void upload(const X* h, size_t count) {
+    alloc(count);
+    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      free();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+};
+
+struct DevRows {
+  DevBuf<long long> row;
+  DevBuf<int32_t> ptr, col;
+  DevBuf<double> val;
+  int nrows = 0;
+  void upload(const SparseRows& s) {
+    nrows = s.nrows();
+    std::vector<long long> r(s.row.begin(), s.row.end());
+    row.upload(r.data(), r.size());
+    ptr.upload(s.ptr.data(), s.ptr.size());
+    col.upload(s.col.data(), s.col.size());
+    val.upload(s.val.data(), s.val.size());
+  }
+};
+
+struct Level {
+  DevGeom g;
+  Stencil st;
+  bool star;
+  int ncolour;
+  int64_t N;
+  // low rank
+  DevBuf<int32_t> bptr;
+  DevBuf<long long> bidx;
+  DevBuf<double> bval;
+  DevRows Brows;
+  DevRows W[2];
+  DevBuf<double> Sinv[2];
+};
+
+struct Common {
+  int K = 1;
+  int m = 0;
+  int nlevel = 1;
+  mgmc_params p{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t k0 = 0, k1 = 0;
+  HostHierarchy H;
+  std::vector<std::unique_ptr<Level>> L;
+  DevBuf<double> sqrt_sigma, sigma_inv_mat, z;
+  DevBuf<long long> cpidx;
+  DevBuf<double> LinvT, Qinv;
+  int64_t Nc = 0;
+  DevBuf<uint32_t> ctr;  // [0] sample counter, [1] observable row
+  DevBuf<long long> probe_pidx;
+  int nprobe = 0;
+  uint32_t tag = 0;
+  // optional HIP-event timing of the level-0 colour-phase launches
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Engine<T>: fields in precision T, everything else shared.
+// ---------------------------------------------------------------------------
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual void cycle_sample() = 0;
+  virtual void cycle_precondition() = 0;
+  virtual void set_fine(int which, const double* ref_dev, int per_chain) = 0;  // 0 = x, 1 = f
+  virtual void get_fine_x(double* ref_dev, double scale) = 0;
+  virtual void probe(double* obs_dev) = 0;
+  virtual void* field(int level, int which) = 0;  // 0 x, 1 f, 2 r
+  virtual void sweep(int level, bool forward, bool stochastic, const double* xi_dev, const double* xilr_dev) = 0;
+  virtual void residual_restrict(int level) = 0;
+  virtual void prolongate(int level, double alpha) = 0;
+  virtual void coarse(int mode, const double* xi_dev) = 0;
+  virtual void pack(int level, int which, const double* ref_dev, int per_chain) = 0;
+  virtual void unpack(int level, int which, double* ref_dev, double scale) = 0;
+  virtual void apply_neg_operator_fp64(const double* x_pad64, double* y_pad64) = 0;
+  virtual void fine_residual_fp64(const double* x_pad64, const double* b_pad64, double* r_pad64) = 0;
+  virtual void set_f_from_fp64(const double* pad64) = 0;
+  virtual void add_x_to_fp64(double* pad64) = 0;
+};
+
+// A level field for all chains with a zeroed guard of GUARD elements on both
+// sides: the stencil kernels read one vertex beyond the aligned 8-vertex group
+// at the first and last row of a chain (values that are never used).
+constexpr size_t GUARD = 64;
+template <typename T>
+struct Field {
+  DevBuf<T> buf;
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count, hipStream_t s) {
+    buf.alloc(count + 2 * GUARD);
+    HIP_CHECK(hipMemsetAsync(buf.p, 0, (count + 2 * GUARD) * sizeof(T), s));
+    p = buf.p + GUARD;
+    n = count;
+  }
+};
+
+template <typename T>
+struct Engine : EngineBase {
+  Common& C;
+  std::vector<Field<T>> x, f, r;
+  Field<T> ext;  // external noise staging (test hooks)
+
+  explicit Engine(Common& c) : C(c) {
+    x.resize(C.nlevel);
+    f.resize(C.nlevel);
+    r.resize(C.nlevel);
+    for (int l = 0; l < C.nlevel; ++l) {
+      size_t n = (size_t)C.L[l]->g.cs * C.K;
+      x[l].alloc(n, C.stream);
+      f[l].alloc(n, C.stream);
+      r[l].alloc(n, C.stream);
+    }
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+  }
+
+  NoiseKey key(uint32_t tag) const { return NoiseKey{C.k0, C.k1, tag, C.ctr.p}; }
+
+  void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, k_fill<T>, p, n, v); }
+
+  // colour phase launch
+  void colour_phase(int l, int colour, int noise, const T* extp, uint32_t tag) {
+    Level& Lv = *C.L[l];
+    const DevGeom& g = Lv.g;
+    int nrows;
+    if (Lv.star) {
+      nrows = (g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1);
+    } else {
+      int jpar = (colour >> 1) & 1, kpar = (colour >> 2) & 1;
+      int nj = jpar ? g.n1 / 2 : (g.n1 - 1) / 2;
+      int nk = g.dim == 3 ? (kpar ? g.n2 / 2 : (g.n2 - 1) / 2) : 1;
+      nrows = nj * nk;
+    }
+    if (nrows <= 0) return;
+    long long threads = (long long)nrows * C.K * (g.px >> 3);
+    dim3 grid(nblocks(threads)), blk(256);
+    NoiseKey nk = key(tag);
+    const bool timed = C.prof && l == 0;
+    if (timed) {
+      if (C.ev_used + 2 > C.ev.size()) {
+        for (int q = 0; q < 64; ++q) {
+          hipEvent_t e;
+          HIP_CHECK(hipEventCreate(&e));
+          C.ev.push_back(e);
+        }
+      }
+      HIP_CHECK(hipEventRecord(C.ev[C.ev_used], C.stream));
+    }
+#define MGMC_SWEEP(DIM, STAR, NOISE) \
+  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, x[l].p, f[l].p, extp, g, Lv.st, colour, nrows, C.K, nk)
+#define MGMC_SWEEP_N(DIM, STAR)                                 \
+  do {                                                          \
+    if (noise == NOISE_NONE) MGMC_SWEEP(DIM, STAR, NOISE_NONE); \
+    else if (noise == NOISE_PHILOX) MGMC_SWEEP(DIM, STAR, NOISE_PHILOX); \
+    else MGMC_SWEEP(DIM, STAR, NOISE_EXTERNAL);                 \
+  } while (0)
+    if (g.dim == 3) {
+      if (Lv.star) MGMC_SWEEP_N(3, true);
+      else MGMC_SWEEP_N(3, false);
+    } else {
+      if (Lv.star) MGMC_SWEEP_N(2, true);
+      else MGMC_SWEEP_N(2, false);
+    }
+#undef MGMC_SWEEP_N
+#undef MGMC_SWEEP
+    if (timed) {
+      HIP_CHECK(hipEventRecord(C.ev[C.ev_used + 1], C.stream));
+      C.ev_used += 2;
+    }
+  }
+
+  void lr_dot(int l, const T* xp) {
+    Level& Lv = *C.L[l];
+    dim3 grid(C.m, C.K);
+    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.z.p);
+  }
+
+  void lr_update(T* xp, long long cs, DevRows& R, const double* M, int noise, const double* extlr, uint32_t tag) {
+    if (R.nrows == 0) return;
+    dim3 grid(nblocks(R.nrows), C.K);
+    NoiseKey nk = key(tag);
+    if (noise == NOISE_NONE)
+      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_NONE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, M,
+                                                             C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+    else if (noise == NOISE_PHILOX)
+      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_PHILOX>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, M,
+                                                               C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+    else
+      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_EXTERNAL>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows,
+                                                                 M, C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+  }
+
+  // One directional SOR sweep (+ SMW), SORSmoother::apply / SORSampler::apply
+  // with nsmooth = 1. `sparse_repeats` > 1 reproduces the nsmooth^2 sparse
+  // sweeps of SORSmoother with nsmooth > 1 (src/smoother/sor_smoother.cc:44,63).
+  void dir_sweep(int l, bool forward, int noise, const T* extp, const double* extlr, int sparse_repeats = 1) {
+    Level& Lv = *C.L[l];
+    uint32_t tag = C.tag++;
+    for (int s = 0; s < sparse_repeats; ++s)
+      for (int q = 0; q < Lv.ncolour; ++q) {
+        int colour = forward ? q : Lv.ncolour - 1 - q;
+        colour_phase(l, colour, s == 0 ? noise : NOISE_NONE, extp, tag);
+      }
+    if (C.m > 0) {
+      lr_dot(l, x[l].p);
+      int d = forward ? 1 : 0;
+      lr_update(x[l].p, Lv.g.cs, Lv.W[d], Lv.Sinv[d].p, noise, extlr, tag);
+    }
+  }
+
+  void pre(int l, bool stoch) {
+    const mgmc_params& p = C.p;
+    int noise = stoch ? NOISE_PHILOX : NOISE_NONE;
+    if (p.smoother == MGMC_SSOR) {
+      for (int s = 0; s < p.npresmooth; ++s) {
+        dir_sweep(l, true, noise, nullptr, nullptr);
+        dir_sweep(l, false, noise, nullptr, nullptr);
+      }
+    } else {
+      // SORSampler: nsmooth fresh-noise sweeps; SORSmoother: nsmooth x (nsmooth sparse + SMW)
+      for (int s = 0; s < p.npresmooth; ++s) dir_sweep(l, true, noise, nullptr, nullptr, stoch ? 1 : p.npresmooth);
+    }
+  }
+  void post(int l, bool stoch) {
+    const mgmc_params& p = C.p;
+    int noise = stoch ? NOISE_PHILOX : NOISE_NONE;
+    if (p.smoother == MGMC_SSOR) {
+      for (int s = 0; s < p.npostsmooth; ++s) {
+        dir_sweep(l, true, noise, nullptr, nullptr);
+        dir_sweep(l, false, noise, nullptr, nullptr);
+      }
+    } else {
+      for (int s = 0; s < p.npostsmooth; ++s) dir_sweep(l, false, noise, nullptr, nullptr, stoch ? 1 : p.npostsmooth);
+    }
+  }
+
+  // f_{l+1} = R (f_l - A_l x_l) - B_{l+1} Sigma^{-1} B_l^T x_l
+  void residual_restrict(int l) override {
+    Level& F = *C.L[l];
+    Level& Cc = *C.L[l + 1];
+    long long nf = F.N * C.K, nc = Cc.N * C.K;
+    if (F.g.dim == 3) {
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, true>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, false>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
+    } else {
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, true>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, false>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
+    }
+    if (C.m > 0) {
+      lr_dot(l, x[l].p);
+      lr_update(f[l + 1].p, Cc.g.cs, Cc.Brows, C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+    }
+  }
+
+  void prolongate(int l, double alpha) override {
+    Level& F = *C.L[l];
+    Level& Cc = *C.L[l + 1];
+    long long nf = F.N * C.K;
+    if (F.g.dim == 3)
+      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 3>, x[l].p, x[l + 1].p, F.g, Cc.g, (T)alpha, C.K);
+    else
+      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 2>, x[l].p, x[l + 1].p, F.g, Cc.g, (T)alpha, C.K);
+  }
+
+  void coarse(int mode, const double* xi_dev) override {
+    int l = C.nlevel - 1;
+    Level& Lv = *C.L[l];
+    size_t sh = 2 * (size_t)C.Nc * sizeof(double);
+    uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
+    NoiseKey nk = key(tag);
+    if (mode == NOISE_NONE)
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+                                                          (int)C.Nc, xi_dev, nk);
+    else if (mode == NOISE_PHILOX)
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+                                                            (int)C.Nc, xi_dev, nk);
+    else
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
+                                                              C.Qinv.p, (int)C.Nc, xi_dev, nk);
+  }
+
+  void zero_x(int l) { fill(x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0); }
+
+  // MultigridMCSampler::sample (multigridmc_sampler.cc:109-136)
+  void sample(int l) {
+    if (l == C.nlevel - 1) {
+      if (C.p.coarse_solver == MGMC_COARSE_CHOLESKY) coarse(NOISE_PHILOX, nullptr);
+      else
+        for (int s = 0; s < C.p.ncoarsesmooth; ++s) {
+          dir_sweep(l, true, NOISE_PHILOX, nullptr, nullptr);
+          dir_sweep(l, false, NOISE_PHILOX, nullptr, nullptr);
+        }
+      return;
+    }
+    int cyc = l > 0 ? C.p.cycle : 1;
+    for (int j = 0; j < cyc; ++j) {
+      pre(l, true);
+      residual_restrict(l);
+      zero_x(l + 1);
+      sample(l + 1);
+      prolongate(l, C.p.coarse_scaling);
+      post(l, true);
+    }
+  }
+
+  // MultigridPreconditioner::solve (multigrid_preconditioner.cc:74-101)
+  void solve(int l) {
+    zero_x(l);
+    if (l == C.nlevel - 1) {
+      coarse(NOISE_NONE, nullptr);
+      return;
+    }
+    int cyc = l > 0 ? C.p.cycle : 1;
+    for (int j = 0; j < cyc; ++j) {
+      pre(l, false);
+      residual_restrict(l);
+      solve(l + 1);
+      prolongate(l, C.p.coarse_scaling);
+      post(l, false);
+    }
+  }
+
+  void cycle_sample() override {
+    C.tag = 1;
+    sample(0);
+  }
+  void cycle_precondition() override {
+    C.tag = 1;
+    solve(0);
+  }
+
+  void pack(int l, int which, const double* ref_dev, int per_chain) override {
+    Level& Lv = *C.L[l];
+    T* dst = which == 0 ? x[l].p : which == 1 ? f[l].p : ext.p;
+    launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_pack<T>, ref_dev, dst, Lv.g, C.K, per_chain);
+  }
+  void unpack(int l, int which, double* ref_dev, double scale) override {
+    Level& Lv = *C.L[l];
+    const T* src = which == 0 ? x[l].p : which == 1 ? f[l].p : r[l].p;
+    launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_unpack<T>, src, ref_dev, Lv.g, C.K, scale);
+  }
+  void set_fine(int which, const double* ref_dev, int per_chain) override { pack(0, which, ref_dev, per_chain); }
+  void get_fine_x(double* ref_dev, double scale) override { unpack(0, 0, ref_dev, scale); }
+  void* field(int l, int which) override { return which == 0 ? x[l].p : which == 1 ? f[l].p : r[l].p; }
+
+  void probe(double* obs_dev) override {
+    if (C.nprobe == 0 || !obs_dev) return;
+    launch(nblocks((long long)C.nprobe * C.K), 256, 0, C.stream, k_probe<T>, x[0].p, C.L[0]->g.cs, C.probe_pidx.p,
+                                                                          C.nprobe, C.K, obs_dev, C.ctr.p);
+  }
+
+  void ensure_ext(int l) {
+    size_t n = (size_t)C.L[l]->g.cs * C.K;
+    // stream-ordered zeroing: the handle's stream does not synchronise with the null stream
+    if (ext.n < n) ext.alloc(n, C.stream);
+  }
+
+  void sweep(int l, bool forward, bool stochastic, const double* xi_dev, const double* xilr_dev) override {
+    int noise = NOISE_NONE;
+    const T* extp = nullptr;
+    if (stochastic) {
+      if (xi_dev) {
+        ensure_ext(l);
+        pack(l, 2, xi_dev, 1);
+        extp = ext.p;
+        noise = NOISE_EXTERNAL;
+      } else {
+        noise = NOISE_PHILOX;
+      }
+    }
+    if (noise == NOISE_EXTERNAL && C.m > 0 && !xilr_dev) throw Error(MGMC_ERR_ARG, "xi_lr needed with injected xi");
+    if (noise == NOISE_PHILOX && xilr_dev) throw Error(MGMC_ERR_ARG, "xi_lr needs xi");
+    dir_sweep(l, forward, noise, extp, xilr_dev);
+  }
+
+  // ---- FP64 fine-level operations for the loop solver (T-independent data) ----
+  // r = b - A x - B Sigma^{-1} B^T x with x, b, r in padded FP64 layout
+  void fine_residual_fp64(const double* x64, const double* b64, double* r64) override {
+    Level& F = *C.L[0];
+    long long nf = F.N * C.K;
+    if (F.g.dim == 3) {
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 3, true>, x64, b64, r64, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 3, false>, x64, b64, r64, F.g, F.st, C.K);
+    } else {
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 2, true>, x64, b64, r64, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 2, false>, x64, b64, r64, F.g, F.st, C.K);
+    }
+    if (C.m > 0) {
+      dim3 grid(C.m, C.K);
+      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.z.p);
+      dim3 g2(nblocks(F.Brows.nrows), C.K);
+      NoiseKey nk = key(0);
+      launch(g2, 256, 0, C.stream, k_lr_update<double, NOISE_NONE>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p,
+                                                                F.Brows.col.p, F.Brows.val.p, F.Brows.nrows,
+                                                                C.sigma_inv_mat.p, C.sqrt_sigma.p, C.z.p, -1.0, C.m,
+                                                                nullptr, nk);
+    }
+  }
+  void apply_neg_operator_fp64(const double* x64, double* y64) override { fine_residual_fp64(x64, nullptr, y64); }
+  void set_f_from_fp64(const double* pad64) override {
+    long long n = (long long)C.L[0]->g.cs * C.K;
+    launch(nblocks(n), 256, 0, C.stream, k_convert<T>, f[0].p, pad64, n);
+  }
+  void add_x_to_fp64(double* pad64) override {
+    long long n = (long long)C.L[0]->g.cs * C.K;
+    launch(nblocks(n), 256, 0, C.stream, k_axpy_pad<T>, pad64, x[0].p, 1.0, n);
+  }
+};
+
+}  // namespace mgmc
+
+using namespace mgmc;
+
+struct mgmc_handle {
+  Common C;
+  std::unique_ptr<EngineBase> E;
+  DevBuf<double> io1, io2, io3;  // reference-order staging
+  DevBuf<double> obs;
+  Field<double> x64, b64, r64;  // loop solver (padded FP64)
+  DevBuf<double> sums;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  double* graph_obs = nullptr;
+  int graph_valid = 0;
+
+  ~mgmc_handle() {
+    for (hipEvent_t e : C.ev) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (C.own_stream && C.stream) (void)hipStreamDestroy(C.stream);
+  }
+};
+
+static int fail(const Error& e) {
+  g_error = e.what();
+  return e.code;
+}
+
+#define API_BEGIN try {
+#define API_END                                       \
+  }                                                   \
+  catch (const Error& e) { return fail(e); }          \
+  catch (const std::exception& e) {                   \
+    g_error = e.what();                               \
+    return MGMC_ERR_ARG;                              \
+  }                                                   \
+  return MGMC_OK;
+
+static void sync(mgmc_handle* h) {
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(h->C.stream));
+}
+
+static double* stage(DevBuf<double>& b, const double* host, size_t n, hipStream_t s) {
+  if (b.n < n) b.alloc(n);
+  if (host) HIP_CHECK(hipMemcpyAsync(b.p, host, n * sizeof(double), hipMemcpyHostToDevice, s));
+  return b.p;
+}
+
+extern "C" {
+
+const char* mgmc_last_error(void) { return g_error.c_str(); }
+
+int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** out) {
+  API_BEGIN
+  if (!prob || !par || !out) throw Error(MGMC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (par->nchains < 1) throw Error(MGMC_ERR_ARG, "nchains must be >= 1");
+  if (par->smoother != MGMC_SOR && par->smoother != MGMC_SSOR) throw Error(MGMC_ERR_ARG, "invalid smoother");
+  if (par->coarse_solver != MGMC_COARSE_CHOLESKY && par->coarse_solver != MGMC_COARSE_SSOR)
+    throw Error(MGMC_ERR_ARG, "invalid coarse solver");
+  if (par->precision != MGMC_FP64 && par->precision != MGMC_FP32) throw Error(MGMC_ERR_ARG, "invalid precision");
+  if (par->cycle < 1) throw Error(MGMC_ERR_ARG, "cycle must be >= 1");
+  if (par->npresmooth < 0 || par->npostsmooth < 0 || par->ncoarsesmooth < 0)
+    throw Error(MGMC_ERR_ARG, "smoothing counts must be >= 0");
+  if (prob->m > MMAX) throw Error(MGMC_ERR_UNSUPPORTED, "at most 128 measurements are supported");
+  auto h = std::make_unique<mgmc_handle>();
+  Common& C = h->C;
+  C.p = *par;
+  C.K = par->nchains;
+  C.m = prob->m;
+  C.nlevel = par->nlevel;
+  C.k0 = (uint32_t)par->seed;
+  C.k1 = (uint32_t)(par->seed >> 32);
+  HostProblem hp{prob->dim, {prob->n[0], prob->n[1], prob->n[2]}, prob->kappa_sq, prob->A_rowptr, prob->A_col,
+                 prob->A_val, prob->m, prob->B_rowptr, prob->B_col, prob->B_val, prob->sigma};
+  C.H = build_hierarchy(hp, par->nlevel, par->omega);
+  if (par->stream) C.stream = (hipStream_t)par->stream;
+  else {
+    HIP_CHECK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    C.own_stream = true;
+  }
+  for (int l = 0; l < C.nlevel; ++l) {
+    const HostLevel& HL = C.H.levels[l];
+    auto Lv = std::make_unique<Level>();
+    Lv->g = DevGeom{HL.g.dim, HL.g.n[0], HL.g.n[1], HL.g.n[2], HL.g.px, HL.g.py, HL.g.pz, (long long)HL.g.cstride};
+    for (int i = 0; i < 27; ++i) Lv->st.c[i] = HL.st[i];
+    Lv->st.diag = HL.diag;
+    Lv->st.omega = par->omega;
+    Lv->st.inv_diag_omega = par->omega / HL.diag;
+    Lv->st.noise_scale = std::sqrt(HL.diag * (2. - par->omega) / par->omega);
+    Lv->star = HL.star;
+    Lv->ncolour = HL.ncolour;
+    Lv->N = HL.g.N;
+    if (C.m > 0) {
+      Lv->bptr.upload(HL.bptr.data(), HL.bptr.size());
+      std::vector<long long> bi(HL.bidx.begin(), HL.bidx.end());
+      Lv->bidx.upload(bi.data(), bi.size());
+      Lv->bval.upload(HL.bval.data(), HL.bval.size());
+      Lv->Brows.upload(HL.Brows);
+      for (int d = 0; d < 2; ++d) {
+        Lv->W[d].upload(HL.W[d]);
+        Lv->Sinv[d].upload(HL.Sinv[d].data(), HL.Sinv[d].size());
+      }
+    }
+    C.L.push_back(std::move(Lv));
+  }
+  if (C.m > 0) {
+    std::vector<double> ss(C.m), sim((size_t)C.m * C.m, 0.0);
+    for (int k = 0; k < C.m; ++k) {
+      ss[k] = std::sqrt(C.H.sigma[k]);
+      sim[(size_t)k * C.m + k] = 1.0 / C.H.sigma[k];
+    }
+    C.sqrt_sigma.upload(ss.data(), ss.size());
+    C.sigma_inv_mat.upload(sim.data(), sim.size());
+    C.z.alloc((size_t)C.m * C.K);
+  }
+  C.Nc = C.H.coarse.Nc;
+  if (C.Nc > 2048) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level has more than 2048 unknowns; add levels");
+  {
+    std::vector<long long> pi(C.H.coarse.pidx.begin(), C.H.coarse.pidx.end());
+    C.cpidx.upload(pi.data(), pi.size());
+    C.LinvT.upload(C.H.coarse.LinvT.data(), C.H.coarse.LinvT.size());
+    C.Qinv.upload(C.H.coarse.Qinv.data(), C.H.coarse.Qinv.size());
+  }
+  uint32_t c0[2] = {0, 0};
+  C.ctr.upload(c0, 2);
+  if (par->precision == MGMC_FP64) h->E = std::make_unique<Engine<double>>(C);
+  else h->E = std::make_unique<Engine<float>>(C);
+  HIP_CHECK(hipDeviceSynchronize());
+  *out = h.release();
+  API_END
+}
+
+int mgmc_destroy(mgmc_handle* h) {
+  API_BEGIN
+  if (h) {
+    (void)hipStreamSynchronize(h->C.stream);
+    delete h;
+  }
+  API_END
+}
+
+int mgmc_ndof(const mgmc_handle* h, int level, int64_t* n) {
+  API_BEGIN
+  if (!h || level < 0 || level >= h->C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  *n = h->C.L[level]->N;
+  API_END
+}
+
+int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours) {
+  API_BEGIN
+  if (!h || level < 0 || level >= h->C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  const HostLevel& HL = h->C.H.levels[level];
+  if (stencil) {
+    if (HL.g.dim == 3) std::copy(HL.st, HL.st + 27, stencil);
+    else std::copy(HL.st + 9, HL.st + 18, stencil);
+  }
+  if (ncolours) *ncolours = HL.ncolour;
+  API_END
+}
+
+int mgmc_set_counter(mgmc_handle* h, uint64_t sample_counter) {
+  API_BEGIN
+  uint32_t c[2] = {(uint32_t)sample_counter, 0};
+  HIP_CHECK(hipMemcpyAsync(h->C.ctr.p, c, sizeof(c), hipMemcpyHostToDevice, h->C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_set_rhs(mgmc_handle* h, const double* f, int f_per_chain) {
+  API_BEGIN
+  int64_t N = h->C.L[0]->N;
+  size_t n = (size_t)N * (f_per_chain ? h->C.K : 1);
+  double* d = stage(h->io1, f, n, h->C.stream);
+  h->E->set_fine(1, d, f_per_chain);
+  sync(h);
+  API_END
+}
+
+int mgmc_set_state(mgmc_handle* h, const double* x) {
+  API_BEGIN
+  int64_t N = h->C.L[0]->N;
+  double* d = stage(h->io1, x, (size_t)N * h->C.K, h->C.stream);
+  h->E->set_fine(0, d, 1);
+  sync(h);
+  API_END
+}
+
+int mgmc_get_state(mgmc_handle* h, double* x) {
+  API_BEGIN
+  int64_t N = h->C.L[0]->N;
+  double* d = stage(h->io1, nullptr, (size_t)N * h->C.K, h->C.stream);
+  h->E->get_fine_x(d, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, d, (size_t)N * h->C.K * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_sample(mgmc_handle* h, const double* f, int f_per_chain, double* x) {
+  API_BEGIN
+  int64_t N = h->C.L[0]->N;
+  double* df = stage(h->io1, f, (size_t)N * (f_per_chain ? h->C.K : 1), h->C.stream);
+  h->E->set_fine(1, df, f_per_chain);
+  double* dx = stage(h->io2, x, (size_t)N * h->C.K, h->C.stream);
+  h->E->set_fine(0, dx, 1);
+  h->E->cycle_sample();
+  launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+  h->E->get_fine_x(dx, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, dx, (size_t)N * h->C.K * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_set_probes(mgmc_handle* h, int nprobe, const int64_t* vertices) {
+  API_BEGIN
+  if (nprobe < 0) throw Error(MGMC_ERR_ARG, "nprobe < 0");
+  const HostLevel& HL = h->C.H.levels[0];
+  std::vector<long long> p(nprobe);
+  for (int i = 0; i < nprobe; ++i) {
+    if (vertices[i] < 0 || vertices[i] >= HL.g.N) throw Error(MGMC_ERR_ARG, "probe vertex out of range");
+    p[i] = HL.g.padded(vertices[i]);
+  }
+  h->C.probe_pidx.upload(p.data(), p.size());
+  h->C.nprobe = nprobe;
+  h->graph_valid = 0;
+  API_END
+}
+
+static void record_cycle(mgmc_handle* h, double* obs_dev) {
+  h->E->cycle_sample();
+  h->E->probe(obs_dev);
+  launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+}
+
+static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph) {
+  if (h->C.prof) use_graph = 0;
+  // observable rows start at 0 for this run
+  HIP_CHECK(hipMemsetAsync(h->C.ctr.p + 1, 0, sizeof(uint32_t), h->C.stream));
+  if (!use_graph) {
+    for (int s = 0; s < nsamples; ++s) record_cycle(h, obs_dev);
+    return;
+  }
+  if (!h->graph_valid || h->graph_obs != obs_dev) {
+    if (h->gexec) HIP_CHECK(hipGraphExecDestroy(h->gexec));
+    if (h->graph) HIP_CHECK(hipGraphDestroy(h->graph));
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(h->C.stream, hipStreamCaptureModeThreadLocal));
+    record_cycle(h, obs_dev);
+    HIP_CHECK(hipStreamEndCapture(h->C.stream, &h->graph));
+    HIP_CHECK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->graph_obs = obs_dev;
+    h->graph_valid = 1;
+  }
+  for (int s = 0; s < nsamples; ++s) HIP_CHECK(hipGraphLaunch(h->gexec, h->C.stream));
+}
+
+int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph) {
+  API_BEGIN
+  if (nsamples < 0) throw Error(MGMC_ERR_ARG, "nsamples < 0");
+  run_dev(h, nsamples, h->C.nprobe ? obs_dev : nullptr, use_graph);
+  HIP_CHECK(hipGetLastError());
+  API_END
+}
+
+int mgmc_run(mgmc_handle* h, int nsamples, double* obs, int use_graph) {
+  API_BEGIN
+  if (nsamples < 0) throw Error(MGMC_ERR_ARG, "nsamples < 0");
+  size_t n = (size_t)nsamples * h->C.K * h->C.nprobe;
+  double* d = nullptr;
+  if (obs && n) {
+    if (h->obs.n < n) {
+      h->obs.alloc(n);
+      h->graph_valid = 0;
+    }
+    d = h->obs.p;
+  }
+  run_dev(h, nsamples, d, use_graph);
+  if (d) HIP_CHECK(hipMemcpyAsync(obs, d, n * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  API_END
+}
+
+int mg_precondition(mgmc_handle* h, const double* b, double* x) {
+  API_BEGIN
+  int64_t N = h->C.L[0]->N;
+  size_t n = (size_t)N * h->C.K;
+  double* db = stage(h->io1, b, n, h->C.stream);
+  h->E->set_fine(1, db, 1);
+  h->E->cycle_precondition();
+  h->E->get_fine_x(db, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, db, n * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_apply_operator(mgmc_handle* h, const double* x, double* y) {
+  API_BEGIN
+  Common& C = h->C;
+  int64_t N = C.L[0]->N;
+  size_t n = (size_t)N * C.K;
+  size_t np = (size_t)C.L[0]->g.cs * C.K;
+  if (h->x64.n < np) {
+    h->x64.alloc(np, C.stream);
+    h->r64.alloc(np, C.stream);
+    h->b64.alloc(np, C.stream);
+  }
+  double* dx = stage(h->io1, x, n, C.stream);
+  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, dx, h->x64.p, C.L[0]->g, C.K, 1);
+  h->E->apply_neg_operator_fp64(h->x64.p, h->r64.p);
+  launch(nblocks(n), 256, 0, C.stream, k_unpack<double>, h->r64.p, dx, C.L[0]->g, C.K, -1.0);
+  HIP_CHECK(hipMemcpyAsync(y, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+// LoopSolver::apply (src/solver/loop_solver.cc:9-54) on all chains; the
+// stopping test and history use chain 0.
+int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
+             int* niter) {
+  API_BEGIN
+  Common& C = h->C;
+  int64_t N = C.L[0]->N;
+  size_t n = (size_t)N * C.K;
+  size_t np = (size_t)C.L[0]->g.cs * C.K;
+  if (h->x64.n < np) {
+    h->x64.alloc(np, C.stream);
+    h->r64.alloc(np, C.stream);
+    h->b64.alloc(np, C.stream);
+  }
+  HIP_CHECK(hipMemsetAsync(h->x64.p, 0, np * sizeof(double), C.stream));
+  HIP_CHECK(hipMemsetAsync(h->r64.p, 0, np * sizeof(double), C.stream));
+  HIP_CHECK(hipMemsetAsync(h->b64.p, 0, np * sizeof(double), C.stream));
+  double* db = stage(h->io1, b, n, C.stream);
+  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, db, h->b64.p, C.L[0]->g, C.K, 1);
+  const int nb = 512;
+  if (h->sums.n < (size_t)nb) h->sums.alloc(nb);
+  std::vector<double> part(nb);
+  long long cs = C.L[0]->g.cs;
+  auto norm0 = [&](const double* v) {
+    launch(nb, 256, 0, C.stream, k_sumsq, v, cs, h->sums.p);
+    HIP_CHECK(hipMemcpyAsync(part.data(), h->sums.p, nb * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+    sync(h);
+    double s = 0;
+    for (double v2 : part) s += v2;
+    return std::sqrt(s);
+  };
+  double r0 = norm0(h->b64.p);
+  int it = -1;
+  for (int k = 0; k < maxiter; ++k) {
+    h->E->fine_residual_fp64(h->x64.p, h->b64.p, h->r64.p);  // r' = b - Q x
+    double rn = norm0(h->r64.p);
+    if (hist) hist[k] = rn;
+    if ((rn / r0 < rtol) && (rn < atol)) {
+      it = k;
+      break;
+    }
+    // x -= MG(Q x - b)  <=>  x += MG(r')
+    h->E->set_f_from_fp64(h->r64.p);
+    h->E->cycle_precondition();
+    h->E->add_x_to_fp64(h->x64.p);
+  }
+  if (niter) *niter = it;
+  launch(nblocks(n), 256, 0, C.stream, k_unpack<double>, h->x64.p, db, C.L[0]->g, C.K, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, db, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  if (it < 0) throw Error(MGMC_ERR_NOCONV, "solver failed to converge after maxiter iterations");
+  API_END
+}
+
+int mgmc_sweep(mgmc_handle* h, int level, int forward, int stochastic, const double* b, double* x, const double* xi,
+               const double* xi_lr) {
+  API_BEGIN
+  Common& C = h->C;
+  if (level < 0 || level >= C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  int64_t N = C.L[level]->N;
+  size_t n = (size_t)N * C.K;
+  double* db = stage(h->io1, b, n, C.stream);
+  h->E->pack(level, 1, db, 1);
+  double* dx = stage(h->io2, x, n, C.stream);
+  h->E->pack(level, 0, dx, 1);
+  double* dxi = xi ? stage(h->io3, xi, n, C.stream) : nullptr;
+  DevBuf<double> lr;
+  if (xi_lr && C.m > 0) lr.upload(xi_lr, (size_t)C.m * C.K);
+  C.tag = 1;
+  h->E->sweep(level, forward != 0, stochastic != 0, dxi, xi_lr && C.m > 0 ? lr.p : nullptr);
+  h->E->unpack(level, 0, dx, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_residual_restrict(mgmc_handle* h, int level, const double* f, const double* x, double* fc) {
+  API_BEGIN
+  Common& C = h->C;
+  if (level < 0 || level + 1 >= C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  size_t n = (size_t)C.L[level]->N * C.K, nc = (size_t)C.L[level + 1]->N * C.K;
+  double* df = stage(h->io1, f, n, C.stream);
+  h->E->pack(level, 1, df, 1);
+  double* dx = stage(h->io2, x, n, C.stream);
+  h->E->pack(level, 0, dx, 1);
+  h->E->residual_restrict(level);
+  double* dc = stage(h->io3, nullptr, nc, C.stream);
+  h->E->unpack(level + 1, 1, dc, 1.0);
+  HIP_CHECK(hipMemcpyAsync(fc, dc, nc * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_prolongate_add(mgmc_handle* h, int level, double alpha, const double* xc, double* x) {
+  API_BEGIN
+  Common& C = h->C;
+  if (level < 0 || level + 1 >= C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  size_t n = (size_t)C.L[level]->N * C.K, nc = (size_t)C.L[level + 1]->N * C.K;
+  double* dc = stage(h->io1, xc, nc, C.stream);
+  h->E->pack(level + 1, 0, dc, 1);
+  double* dx = stage(h->io2, x, n, C.stream);
+  h->E->pack(level, 0, dx, 1);
+  h->E->prolongate(level, alpha);
+  h->E->unpack(level, 0, dx, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_coarse(mgmc_handle* h, const double* f, double* x, int xi_mode, const double* xi) {
+  API_BEGIN
+  Common& C = h->C;
+  int l = C.nlevel - 1;
+  size_t n = (size_t)C.L[l]->N * C.K;
+  if (xi_mode < 0 || xi_mode > 2) throw Error(MGMC_ERR_ARG, "invalid xi_mode");
+  if (xi_mode == 2 && !xi) throw Error(MGMC_ERR_ARG, "xi_mode 2 needs xi");
+  double* df = stage(h->io1, f, n, C.stream);
+  h->E->pack(l, 1, df, 1);
+  double* dxi = xi_mode == 2 ? stage(h->io3, xi, n, C.stream) : nullptr;
+  C.tag = 1;
+  h->E->coarse(xi_mode, dxi);
+  double* dx = stage(h->io2, nullptr, n, C.stream);
+  h->E->unpack(l, 0, dx, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out) {
+  API_BEGIN
+  Common& C = h->C;
+  size_t n = (size_t)C.L[0]->N * C.K;
+  double* d = stage(h->io1, nullptr, n, C.stream);
+  NoiseKey nk{C.k0, C.k1, tag, C.ctr.p};
+  launch(nblocks(n), 256, 0, C.stream, k_normals, d, C.L[0]->g, C.K, nk);
+  HIP_CHECK(hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+int mgmc_profile(mgmc_handle* h, int enable) {
+  API_BEGIN
+  h->C.prof = enable != 0;
+  h->C.ev_used = 0;
+  h->graph_valid = 0;
+  API_END
+}
+
+int mgmc_profile_result(mgmc_handle* h, double* avg_ms, int64_t* count) {
+  API_BEGIN
+  sync(h);
+  double tot = 0.0;
+  for (size_t i = 0; i + 1 < h->C.ev_used; i += 2) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, h->C.ev[i], h->C.ev[i + 1]));
+    tot += ms;
+  }
+  int64_t n = (int64_t)(h->C.ev_used / 2);
+  if (count) *count = n;
+  if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+  h->C.ev_used = 0;
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,509 @@
+// Host-side (FP64) construction of the multigrid hierarchy.
+//
+// Reference behaviour reproduced here (paths relative to the reference root):
+//  * fine operator: ShiftedLaplaceFDOperator (src/linear_operator/
+//    shiftedlaplace_fd_operator.cc:9-57) or a caller-supplied CSR matrix,
+//    recognised as a constant nearest-neighbour stencil,
+//  * Galerkin coarsening A_c = R A R^T, B_c = R B, Sigma_c = Sigma
+//    (src/linear_operator/linear_operator.cc:10-23) with R = P^T and P the
+//    tensor product of {1/2, 1, 1/2} (src/intergrid/intergrid_operator_linear.cc:8-31),
+//  * the SMW low-rank correction of SORSmoother (src/smoother/sor_smoother.cc:17-37),
+//    kept factored as W = (L + D/omega)^{-1} B (sparse) and S^{-1} = (Sigma + B^T W)^{-1}
+//    instead of the dense N x m matrix B W S^{-1},
+//  * the coarsest-level Cholesky factor (src/sampler/cholesky_sampler.cc:9-23,
+//    src/solver/cholesky_solver.cc:8-27), stored as explicit L^{-T} and Q^{-1}
+//    so that the per-cycle triangular solves become dense products.
+#include "setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <unordered_map>
+
+#include "../../include/mgmc_amd.h"
+
+namespace mgmc {
+
+static int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+Geom Geom::make(int dim, const int* n) {
+  if (dim != 2 && dim != 3) throw Error(MGMC_ERR_UNSUPPORTED, "only dim = 2 or 3 is supported");
+  Geom g;
+  g.dim = dim;
+  for (int d = 0; d < 3; ++d) g.n[d] = (d < dim) ? n[d] : 1;
+  for (int d = 0; d < dim; ++d)
+    if (g.n[d] < 2) throw Error(MGMC_ERR_ARG, "lattice extent must be >= 2 cells");
+  g.px = (int)round_up(g.n[0] + 1, 8);
+  g.py = g.n[1] + 1;
+  g.pz = dim == 3 ? g.n[2] + 1 : 1;
+  g.vol = (int64_t)g.px * g.py * g.pz;
+  g.cstride = round_up(g.vol, 64);
+  g.N = (int64_t)(g.n[0] - 1) * (g.n[1] - 1) * (dim == 3 ? g.n[2] - 1 : 1);
+  if (g.vol >= (int64_t(1) << 31)) throw Error(MGMC_ERR_UNSUPPORTED, "level too large for 32-bit row ids");
+  return g;
+}
+
+// Lattice3d::get_coarse_lattice (src/lattice/lattice3d.hh:248-263)
+Geom Geom::coarse() const {
+  int nc[3];
+  for (int d = 0; d < dim; ++d) {
+    if (n[d] % 2 != 0) throw Error(MGMC_ERR_ARG, "cannot coarsen lattice [one of the extents is odd]");
+    if (!(n[d] / 2 > 1)) throw Error(MGMC_ERR_ARG, "cannot coarsen lattice [no interior vertices]");
+    nc[d] = n[d] / 2;
+  }
+  return make(dim, nc);
+}
+
+void Geom::euclid(int64_t ell, int* idx) const {
+  idx[0] = (int)(ell % (n[0] - 1)) + 1;
+  ell /= (n[0] - 1);
+  idx[1] = (int)(ell % (n[1] - 1)) + 1;
+  ell /= (n[1] - 1);
+  idx[2] = dim == 3 ? (int)ell + 1 : 0;
+}
+
+int64_t Geom::padded(int64_t ell) const {
+  int e[3];
+  euclid(ell, e);
+  return ((int64_t)e[2] * py + e[1]) * px + e[0];
+}
+
+static inline int sidx(int dx, int dy, int dz) { return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1); }
+
+// RAP of a constant stencil: s_c(D) = sum_{sigma,tau} p(sigma) p(tau) s(2D + tau - sigma)
+void galerkin_stencil(int dim, const double* s, double* c) {
+  const double w1[3] = {0.5, 1.0, 0.5};
+  int zr = dim == 3 ? 1 : 0;
+  for (int i = 0; i < 27; ++i) c[i] = 0.0;
+  for (int Dz = -zr; Dz <= zr; ++Dz)
+    for (int Dy = -1; Dy <= 1; ++Dy)
+      for (int Dx = -1; Dx <= 1; ++Dx) {
+        double acc = 0.0;
+        for (int sz = -zr; sz <= zr; ++sz)
+          for (int sy = -1; sy <= 1; ++sy)
+            for (int sx = -1; sx <= 1; ++sx)
+              for (int tz = -zr; tz <= zr; ++tz)
+                for (int ty = -1; ty <= 1; ++ty)
+                  for (int tx = -1; tx <= 1; ++tx) {
+                    int ox = 2 * Dx + tx - sx, oy = 2 * Dy + ty - sy, oz = 2 * Dz + tz - sz;
+                    if (ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1) continue;
+                    double p = w1[sx + 1] * w1[sy + 1] * w1[tx + 1] * w1[ty + 1];
+                    if (dim == 3) p *= w1[sz + 1] * w1[tz + 1];
+                    acc += p * s[sidx(ox, oy, oz)];
+                  }
+        c[sidx(Dx, Dy, Dz)] = acc;
+      }
+}
+
+static void classify(HostLevel& L) {
+  L.star = true;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int nz = (dx != 0) + (dy != 0) + (dz != 0);
+        if (nz > 1 && L.st[sidx(dx, dy, dz)] != 0.0) L.star = false;
+      }
+  L.ncolour = L.star ? 2 : (1 << L.g.dim);
+  L.diag = L.st[sidx(0, 0, 0)];
+  if (!(L.diag > 0.0)) throw Error(MGMC_ERR_UNSUPPORTED, "stencil diagonal must be positive");
+}
+
+// Fine stencil from ShiftedLaplaceFDOperator with constant kappa^2
+// (shiftedlaplace_fd_operator.cc:21-56; the diagonal accumulates in the same order).
+static void fd_stencil(const Geom& g, double kappa_sq, double* st) {
+  for (int i = 0; i < 27; ++i) st[i] = 0.0;
+  double hinv2[3], cv = 1.0;
+  for (int d = 0; d < g.dim; ++d) {
+    double h = 1. / double(g.n[d]);
+    hinv2[d] = 1. / (h * h);
+    cv *= h;
+  }
+  double diagonal = cv * kappa_sq;
+  for (int d = 0; d < g.dim; ++d) {
+    int o[3] = {0, 0, 0};
+    o[d] = -1;
+    st[sidx(o[0], o[1], o[2])] = -cv * hinv2[d];
+    o[d] = 1;
+    st[sidx(o[0], o[1], o[2])] = -cv * hinv2[d];
+    diagonal += 2. * cv * hinv2[d];
+  }
+  st[sidx(0, 0, 0)] = diagonal;
+}
+
+// Recognise a caller-supplied CSR matrix as a constant stencil with
+// Dirichlet elimination; anything else is reported as unsupported.
+static void stencil_from_csr(const Geom& g, const int64_t* rp, const int64_t* ci, const double* v, double* st) {
+  for (int i = 0; i < 27; ++i) st[i] = 0.0;
+  int zr = g.dim == 3 ? 1 : 0;
+  // first pass: a row whose whole 3^d neighbourhood is interior defines the stencil
+  int64_t ref_row = -1;
+  for (int64_t r = 0; r < g.N && ref_row < 0; ++r) {
+    int e[3];
+    g.euclid(r, e);
+    bool deep = e[0] > 1 && e[0] < g.n[0] - 1 && e[1] > 1 && e[1] < g.n[1] - 1;
+    if (g.dim == 3) deep = deep && e[2] > 1 && e[2] < g.n[2] - 1;
+    if (deep) ref_row = r;
+  }
+  if (ref_row < 0) ref_row = 0;
+  double scale = 0.0;
+  for (int64_t r = 0; r < g.N; ++r) {
+    int a[3];
+    g.euclid(r, a);
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      int b[3];
+      if (ci[k] < 0 || ci[k] >= g.N) throw Error(MGMC_ERR_ARG, "A column index out of range");
+      g.euclid(ci[k], b);
+      int o[3] = {b[0] - a[0], b[1] - a[1], g.dim == 3 ? b[2] - a[2] : 0};
+      for (int d = 0; d < 3; ++d)
+        if (o[d] < -1 || o[d] > 1) throw Error(MGMC_ERR_UNSUPPORTED, "A couples vertices that are not neighbours");
+      if (r == ref_row) {
+        st[sidx(o[0], o[1], o[2])] = v[k];
+        scale = std::max(scale, std::fabs(v[k]));
+      }
+    }
+  }
+  (void)zr;
+  // second pass: every row must be the stencil restricted to interior neighbours
+  double tol = 1e-11 * scale;
+  for (int64_t r = 0; r < g.N; ++r) {
+    int a[3];
+    g.euclid(r, a);
+    double seen[27] = {0};
+    bool hit[27] = {false};
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      int b[3];
+      g.euclid(ci[k], b);
+      int s = sidx(b[0] - a[0], b[1] - a[1], g.dim == 3 ? b[2] - a[2] : 0);
+      seen[s] += v[k];
+      hit[s] = true;
+    }
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (g.dim == 2 && dz != 0) continue;
+          int s = sidx(dx, dy, dz);
+          int b[3] = {a[0] + dx, a[1] + dy, a[2] + dz};
+          bool inside = b[0] >= 1 && b[0] <= g.n[0] - 1 && b[1] >= 1 && b[1] <= g.n[1] - 1;
+          if (g.dim == 3) inside = inside && b[2] >= 1 && b[2] <= g.n[2] - 1;
+          double want = inside ? st[s] : 0.0;
+          if (std::fabs(seen[s] - want) > tol || (hit[s] && !inside))
+            throw Error(MGMC_ERR_UNSUPPORTED, "A is not a constant-coefficient stencil (row " + std::to_string(r) + ")");
+        }
+  }
+  for (int s = 0; s < 27; ++s) {
+    int dz = s / 9 - 1, dy = (s / 3) % 3 - 1, dx = s % 3 - 1;
+    int t = sidx(-dx, -dy, -dz);
+    if (std::fabs(st[s] - st[t]) > tol) throw Error(MGMC_ERR_UNSUPPORTED, "A is not symmetric");
+  }
+}
+
+static int colour_of(const HostLevel& L, int i, int j, int k) {
+  if (L.star) return (i + j + k) & 1;
+  return (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+}
+
+// Gauss-Jordan inverse with partial pivoting (m x m row-major)
+static std::vector<double> inverse(std::vector<double> A, int m) {
+  std::vector<double> I(m * m, 0.0);
+  for (int i = 0; i < m; ++i) I[i * m + i] = 1.0;
+  for (int c = 0; c < m; ++c) {
+    int p = c;
+    for (int r = c + 1; r < m; ++r)
+      if (std::fabs(A[r * m + c]) > std::fabs(A[p * m + c])) p = r;
+    if (A[p * m + c] == 0.0) throw Error(MGMC_ERR_NOT_SPD, "singular SMW capacitance matrix");
+    if (p != c)
+      for (int k = 0; k < m; ++k) {
+        std::swap(A[p * m + k], A[c * m + k]);
+        std::swap(I[p * m + k], I[c * m + k]);
+      }
+    double d = 1.0 / A[c * m + c];
+    for (int k = 0; k < m; ++k) {
+      A[c * m + k] *= d;
+      I[c * m + k] *= d;
+    }
+    for (int r = 0; r < m; ++r) {
+      if (r == c) continue;
+      double f = A[r * m + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < m; ++k) {
+        A[r * m + k] -= f * A[c * m + k];
+        I[r * m + k] -= f * I[c * m + k];
+      }
+    }
+  }
+  return I;
+}
+
+// Column k of B_l: (padded index, value) sorted by index
+using Column = std::vector<std::pair<int64_t, double>>;
+
+static void set_columns(HostLevel& L, const std::vector<Column>& cols) {
+  L.bptr.assign(1, 0);
+  L.bidx.clear();
+  L.bval.clear();
+  std::map<int64_t, std::vector<std::pair<int, double>>> rows;
+  for (size_t k = 0; k < cols.size(); ++k) {
+    for (auto& e : cols[k]) {
+      L.bidx.push_back(e.first);
+      L.bval.push_back(e.second);
+      rows[e.first].push_back({(int)k, e.second});
+    }
+    L.bptr.push_back((int32_t)L.bidx.size());
+  }
+  SparseRows& R = L.Brows;
+  R = SparseRows();
+  for (auto& kv : rows) {
+    R.row.push_back(kv.first);
+    for (auto& e : kv.second) {
+      R.col.push_back(e.first);
+      R.val.push_back(e.second);
+    }
+    R.ptr.push_back((int32_t)R.col.size());
+  }
+}
+
+// B_{l+1} = R B_l (scatter form of src/intergrid/intergrid_operator.hh:74-88)
+static std::vector<Column> restrict_columns(const Geom& f, const Geom& c, const std::vector<Column>& cols) {
+  std::vector<Column> out(cols.size());
+  for (size_t k = 0; k < cols.size(); ++k) {
+    std::map<int64_t, double> acc;
+    for (auto& e : cols[k]) {
+      int64_t p = e.first;
+      int i = (int)(p % f.px);
+      int64_t q = p / f.px;
+      int j = (int)(q % f.py);
+      int kz = (int)(q / f.py);
+      int I0[3], I1[3];
+      double w0[3], w1[3];
+      int cnt[3];
+      int e3[3] = {i, j, kz};
+      for (int d = 0; d < 3; ++d) {
+        if (d >= f.dim) {
+          I0[d] = 0; w0[d] = 1.0; cnt[d] = 1;
+          continue;
+        }
+        if (e3[d] % 2 == 0) {
+          I0[d] = e3[d] / 2; w0[d] = 1.0; cnt[d] = 1;
+        } else {
+          I0[d] = (e3[d] - 1) / 2; w0[d] = 0.5;
+          I1[d] = (e3[d] + 1) / 2; w1[d] = 0.5; cnt[d] = 2;
+        }
+      }
+      for (int a = 0; a < cnt[0]; ++a)
+        for (int b = 0; b < cnt[1]; ++b)
+          for (int g = 0; g < cnt[2]; ++g) {
+            int ci = a ? I1[0] : I0[0], cj = b ? I1[1] : I0[1], ck = g ? I1[2] : I0[2];
+            double w = (a ? w1[0] : w0[0]) * (b ? w1[1] : w0[1]) * (g ? w1[2] : w0[2]);
+            if (ci < 1 || ci > c.n[0] - 1 || cj < 1 || cj > c.n[1] - 1) continue;
+            if (c.dim == 3 && (ck < 1 || ck > c.n[2] - 1)) continue;
+            acc[((int64_t)ck * c.py + cj) * c.px + ci] += w * e.second;
+          }
+    }
+    for (auto& kv : acc) out[k].push_back(kv);
+  }
+  return out;
+}
+
+// W = T^{-1} B with T = D/omega + couplings to earlier colours (the
+// triangular factor of SORSmoother for the multicolour visiting order).
+static void compute_lowrank(HostLevel& L, const std::vector<Column>& cols, const std::vector<double>& sigma,
+                            double omega) {
+  const Geom& g = L.g;
+  int m = (int)cols.size();
+  int zr = g.dim == 3 ? 1 : 0;
+  for (int dir = 0; dir < 2; ++dir) {
+    std::map<int64_t, std::vector<std::pair<int, double>>> rows;
+    for (int k = 0; k < m; ++k) {
+      int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+      for (auto& e : cols[k]) {
+        int64_t p = e.first;
+        int c3[3] = {(int)(p % g.px), (int)((p / g.px) % g.py), (int)(p / ((int64_t)g.px * g.py))};
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = std::min(lo[d], c3[d]);
+          hi[d] = std::max(hi[d], c3[d]);
+        }
+      }
+      if (cols[k].empty()) continue;
+      int grow = L.ncolour;
+      int b0[3], b1[3];
+      for (int d = 0; d < 3; ++d) {
+        int nmax = d < g.dim ? g.n[d] - 1 : 0;
+        int nmin = d < g.dim ? 1 : 0;
+        b0[d] = std::max(nmin, lo[d] - grow);
+        b1[d] = std::min(nmax, hi[d] + grow);
+      }
+      int ex = b1[0] - b0[0] + 3, ey = b1[1] - b0[1] + 3, ez = b1[2] - b0[2] + 1 + 2 * zr;
+      auto bi = [&](int i, int j, int kk) -> int64_t {
+        return ((int64_t)(kk - b0[2] + zr) * ey + (j - b0[1] + 1)) * ex + (i - b0[0] + 1);
+      };
+      std::vector<double> bb((size_t)ex * ey * ez, 0.0), w((size_t)ex * ey * ez, 0.0);
+      for (auto& e : cols[k]) {
+        int64_t p = e.first;
+        bb[bi((int)(p % g.px), (int)((p / g.px) % g.py), (int)(p / ((int64_t)g.px * g.py)))] += e.second;
+      }
+      for (int s = 0; s < L.ncolour; ++s) {
+        int col = dir == 1 ? s : L.ncolour - 1 - s;
+        for (int kk = b0[2]; kk <= b1[2]; ++kk)
+          for (int j = b0[1]; j <= b1[1]; ++j)
+            for (int i = b0[0]; i <= b1[0]; ++i) {
+              if (colour_of(L, i, j, kk) != col) continue;
+              double acc = bb[bi(i, j, kk)];
+              for (int dz = -zr; dz <= zr; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                  for (int dx = -1; dx <= 1; ++dx) {
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    double a = L.st[sidx(dx, dy, dz)];
+                    if (a != 0.0) acc -= a * w[bi(i + dx, j + dy, kk + dz)];
+                  }
+              w[bi(i, j, kk)] = acc / (L.diag / omega);
+            }
+      }
+      for (int kk = b0[2]; kk <= b1[2]; ++kk)
+        for (int j = b0[1]; j <= b1[1]; ++j)
+          for (int i = b0[0]; i <= b1[0]; ++i) {
+            double v = w[bi(i, j, kk)];
+            if (v != 0.0) rows[((int64_t)kk * g.py + j) * g.px + i].push_back({k, v});
+          }
+    }
+    SparseRows& W = L.W[dir];
+    W = SparseRows();
+    std::unordered_map<int64_t, int> where;
+    for (auto& kv : rows) {
+      where[kv.first] = W.nrows();
+      W.row.push_back(kv.first);
+      for (auto& e : kv.second) {
+        W.col.push_back(e.first);
+        W.val.push_back(e.second);
+      }
+      W.ptr.push_back((int32_t)W.col.size());
+    }
+    // S = Sigma + B^T W
+    std::vector<double> S((size_t)m * m, 0.0);
+    for (int k = 0; k < m; ++k) {
+      for (auto& e : cols[k]) {
+        auto it = where.find(e.first);
+        if (it == where.end()) continue;
+        int r = it->second;
+        for (int a = W.ptr[r]; a < W.ptr[r + 1]; ++a) S[(size_t)k * m + W.col[a]] += e.second * W.val[a];
+      }
+      S[(size_t)k * m + k] += sigma[k];
+    }
+    L.Sinv[dir] = inverse(S, m);
+  }
+}
+
+static void build_coarse(const HostLevel& L, const std::vector<Column>& cols, const std::vector<double>& sigma,
+                         HostCoarse& C) {
+  const Geom& g = L.g;
+  int64_t Nc = g.N;
+  if (Nc > 8192) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level too large for the dense Cholesky sampler");
+  C.Nc = Nc;
+  C.pidx.resize(Nc);
+  std::unordered_map<int64_t, int64_t> ref;
+  for (int64_t r = 0; r < Nc; ++r) {
+    C.pidx[r] = g.padded(r);
+    ref[C.pidx[r]] = r;
+  }
+  std::vector<double> Q((size_t)Nc * Nc, 0.0);
+  int zr = g.dim == 3 ? 1 : 0;
+  for (int64_t r = 0; r < Nc; ++r) {
+    int e[3];
+    g.euclid(r, e);
+    for (int dz = -zr; dz <= zr; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int b[3] = {e[0] + dx, e[1] + dy, e[2] + dz};
+          if (b[0] < 1 || b[0] > g.n[0] - 1 || b[1] < 1 || b[1] > g.n[1] - 1) continue;
+          if (g.dim == 3 && (b[2] < 1 || b[2] > g.n[2] - 1)) continue;
+          int64_t p = ((int64_t)b[2] * g.py + b[1]) * g.px + b[0];
+          Q[(size_t)r * Nc + ref[p]] += L.st[sidx(dx, dy, dz)];
+        }
+  }
+  for (size_t k = 0; k < cols.size(); ++k)
+    for (auto& a : cols[k])
+      for (auto& b : cols[k]) Q[(size_t)ref[a.first] * Nc + ref[b.first]] += a.second * b.second / sigma[k];
+  // Cholesky Q = L L^T
+  std::vector<double>& A = Q;
+  for (int64_t j = 0; j < Nc; ++j) {
+    double s = A[j * Nc + j];
+    for (int64_t k = 0; k < j; ++k) s -= A[j * Nc + k] * A[j * Nc + k];
+    if (!(s > 0.0)) throw Error(MGMC_ERR_NOT_SPD, "coarse precision matrix is not positive definite");
+    double d = std::sqrt(s);
+    A[j * Nc + j] = d;
+    for (int64_t i = j + 1; i < Nc; ++i) {
+      double t = A[i * Nc + j];
+      for (int64_t k = 0; k < j; ++k) t -= A[i * Nc + k] * A[j * Nc + k];
+      A[i * Nc + j] = t / d;
+    }
+  }
+  // Linv = L^{-1} (lower triangular), column by column
+  std::vector<double> Linv((size_t)Nc * Nc, 0.0);
+  for (int64_t c = 0; c < Nc; ++c)
+    for (int64_t i = c; i < Nc; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int64_t k = c; k < i; ++k) s -= A[i * Nc + k] * Linv[k * Nc + c];
+      Linv[i * Nc + c] = s / A[i * Nc + i];
+    }
+  C.LinvT.assign((size_t)Nc * Nc, 0.0);
+  for (int64_t i = 0; i < Nc; ++i)
+    for (int64_t j = 0; j < Nc; ++j) C.LinvT[i * Nc + j] = Linv[j * Nc + i];
+  C.Qinv.assign((size_t)Nc * Nc, 0.0);
+  for (int64_t i = 0; i < Nc; ++i)
+    for (int64_t j = 0; j < Nc; ++j) {
+      double s = 0.0;
+      for (int64_t k = std::max(i, j); k < Nc; ++k) s += Linv[k * Nc + i] * Linv[k * Nc + j];
+      C.Qinv[i * Nc + j] = s;
+    }
+}
+
+HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega) {
+  if (nlevel < 1) throw Error(MGMC_ERR_ARG, "nlevel must be >= 1");
+  if (!(omega > 0.0 && omega < 2.0)) throw Error(MGMC_ERR_ARG, "omega must lie in (0, 2)");
+  if (p.m < 0) throw Error(MGMC_ERR_ARG, "m must be >= 0");
+  HostHierarchy H;
+  H.omega = omega;
+  H.sigma.assign(p.sigma ? p.sigma : nullptr, p.sigma ? p.sigma + p.m : nullptr);
+  for (int k = 0; k < p.m; ++k)
+    if (!(H.sigma[k] > 0.0)) throw Error(MGMC_ERR_ARG, "measurement variances must be positive");
+  Geom g = Geom::make(p.dim, p.n);
+  HostLevel L0;
+  L0.g = g;
+  if (p.A_rowptr) stencil_from_csr(g, p.A_rowptr, p.A_col, p.A_val, L0.st);
+  else {
+    if (!(p.kappa_sq >= 0.0)) throw Error(MGMC_ERR_ARG, "kappa^2 must be >= 0");
+    fd_stencil(g, p.kappa_sq, L0.st);
+  }
+  classify(L0);
+  std::vector<Column> cols(p.m);
+  if (p.m > 0) {
+    if (!p.B_rowptr) throw Error(MGMC_ERR_ARG, "m > 0 needs B");
+    for (int64_t r = 0; r < g.N; ++r)
+      for (int64_t k = p.B_rowptr[r]; k < p.B_rowptr[r + 1]; ++k) {
+        if (p.B_col[k] < 0 || p.B_col[k] >= p.m) throw Error(MGMC_ERR_ARG, "B column index out of range");
+        cols[p.B_col[k]].push_back({g.padded(r), p.B_val[k]});
+      }
+    for (auto& c : cols) std::sort(c.begin(), c.end());
+  }
+  H.levels.push_back(L0);
+  for (int l = 1; l < nlevel; ++l) {
+    HostLevel& F = H.levels.back();
+    HostLevel C;
+    C.g = F.g.coarse();
+    galerkin_stencil(C.g.dim, F.st, C.st);
+    classify(C);
+    H.levels.push_back(C);
+  }
+  for (int l = 0; l < nlevel; ++l) {
+    HostLevel& L = H.levels[l];
+    L.m = p.m;
+    if (l > 0 && p.m > 0) cols = restrict_columns(H.levels[l - 1].g, L.g, cols);
+    if (p.m > 0) {
+      set_columns(L, cols);
+      compute_lowrank(L, cols, H.sigma, omega);
+    }
+    if (l == nlevel - 1) build_coarse(L, cols, H.sigma, H.coarse);
+  }
+  return H;
+}
+
+}  // namespace mgmc

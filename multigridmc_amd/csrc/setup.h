@@ -1,0 +1,102 @@
+// Host-side (FP64) construction of the multigrid hierarchy the kernels run on.
+//
+// Device data layout. Level l of a lattice with n_d cells per direction keeps
+// every field as a zero-padded box that includes the Dirichlet boundary
+// vertices: index (k * py + j) * px + i with 0 <= i <= n_x etc., pitch
+// px = round_up(n_x + 1, 8) (aligned 8-vertex groups for the noise and the
+// vector loads), py = n_y + 1, pz = n_z + 1 (1 in 2D). Boundary and padding
+// entries stay zero, so a constant stencil applied everywhere equals the
+// reference's operator with eliminated boundary rows. K chains are stored
+// back to back with a stride rounded up to 64 elements.
+#ifndef MGMC_SETUP_H
+#define MGMC_SETUP_H
+
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mgmc {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Geom {
+  int dim = 3;
+  int n[3] = {1, 1, 1};
+  int px = 0, py = 0, pz = 0;
+  int64_t vol = 0;     // px * py * pz
+  int64_t cstride = 0; // per-chain stride (elements)
+  int64_t N = 0;       // interior unknowns
+
+  static Geom make(int dim, const int* n);
+  Geom coarse() const;
+  // reference vertex index -> padded index
+  int64_t padded(int64_t ell) const;
+  void euclid(int64_t ell, int* idx) const;
+};
+
+// Sparse rows: x[row[r]] += sum_{e in [ptr[r], ptr[r+1])} val[e] * u[col[e]]
+struct SparseRows {
+  std::vector<int64_t> row;
+  std::vector<int32_t> ptr{0};
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  int nrows() const { return (int)row.size(); }
+};
+
+struct HostLevel {
+  Geom g;
+  bool star = true;      // 5/7-point stencil -> red-black; else 2^d colours
+  int ncolour = 2;
+  double st[27] = {0};   // stencil, index (dz+1)*9 + (dy+1)*3 + (dx+1)
+  double diag = 0;
+  int m = 0;
+  // columns of B_l as (padded index, value), used for z = B^T x
+  std::vector<int32_t> bptr{0};
+  std::vector<int64_t> bidx;
+  std::vector<double> bval;
+  SparseRows Brows;      // B_l by rows (coarse-side term of the residual)
+  SparseRows W[2];       // W = (L_colour + D/omega)^{-1} B, [0] backward, [1] forward
+  std::vector<double> Sinv[2];  // (Sigma + B^T W)^{-1}, m x m row-major
+};
+
+struct HostCoarse {
+  int64_t Nc = 0;
+  std::vector<int64_t> pidx;    // padded index of coarse vertex (reference order)
+  std::vector<double> LinvT;    // L^{-T}, Nc x Nc row-major
+  std::vector<double> Qinv;     // Q_c^{-1} = L^{-T} L^{-1}
+};
+
+struct HostProblem {
+  int dim;
+  int n[3];
+  double kappa_sq;
+  const int64_t* A_rowptr;
+  const int64_t* A_col;
+  const double* A_val;
+  int m;
+  const int64_t* B_rowptr;
+  const int64_t* B_col;
+  const double* B_val;
+  const double* sigma;
+};
+
+struct HostHierarchy {
+  std::vector<HostLevel> levels;
+  HostCoarse coarse;
+  std::vector<double> sigma;
+  double omega = 1.0;
+};
+
+HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega);
+
+// Galerkin coarse stencil of a constant fine stencil with linear interpolation.
+void galerkin_stencil(int dim, const double* fine, double* coarse);
+
+}  // namespace mgmc
+
+#endif

@@ -1,0 +1,236 @@
+"""Host-side operator setup mirroring the reference's lattice and operator classes.
+
+Only setup lives here (NumPy/SciPy, FP64): lattices, the shifted-Laplace FD
+prior and the ball-average measurement matrix B. Everything that runs per
+sample is in the HIP library.
+"""
+import math
+
+import numpy as np
+import scipy.sparse as sp
+
+
+class Lattice:
+    """Structured lattice of the unit square/cube with n[d] cells per direction;
+    unknowns on the interior vertices, x fastest (src/lattice/lattice3d.hh:60-290)."""
+
+    def __init__(self, *n):
+        if len(n) not in (2, 3):
+            raise ValueError("2D or 3D lattices only")
+        self.n = tuple(int(v) for v in n)
+        self.dim = len(n)
+
+    @property
+    def Nvertex(self):
+        return int(np.prod([v - 1 for v in self.n]))
+
+    @property
+    def Ncell(self):
+        return int(np.prod(self.n))
+
+    def cell_volume(self):
+        return float(np.prod([1.0 / v for v in self.n]))
+
+    def vertex_euclidean(self, ell):
+        """1-based Euclidean vertex indices of linear vertex indices (array)."""
+        ell = np.asarray(ell, dtype=np.int64)
+        out = []
+        for v in self.n:
+            out.append(ell % (v - 1) + 1)
+            ell = ell // (v - 1)
+        return np.stack(out, axis=-1)
+
+    def vertex_coordinates(self, ell):
+        return self.vertex_euclidean(ell) * (1.0 / np.array(self.n, dtype=np.float64))
+
+    def vertex_index(self, idx):
+        idx = np.asarray(idx, dtype=np.int64)
+        ell = np.zeros(idx.shape[:-1], dtype=np.int64)
+        stride = 1
+        for d, v in enumerate(self.n):
+            ell += (idx[..., d] - 1) * stride
+            stride *= v - 1
+        return ell
+
+    def get_coarse_lattice(self):
+        for v in self.n:
+            if v % 2 != 0:
+                raise ValueError("cannot coarsen lattice [one of the extents is odd]")
+            if not v // 2 > 1:
+                raise ValueError("cannot coarsen lattice [resulting lattice would have no interior vertices]")
+        return Lattice(*[v // 2 for v in self.n])
+
+
+def Lattice2d(nx, ny):
+    return Lattice(nx, ny)
+
+
+def Lattice3d(nx, ny, nz):
+    return Lattice(nx, ny, nz)
+
+
+class ShiftedLaplaceFDOperator:
+    """Prior precision h^d (-Delta_h + kappa^2) with Dirichlet elimination
+    (src/linear_operator/shiftedlaplace_fd_operator.cc:9-57), constant
+    kappa^2 = 1 / Lambda^2 (src/linear_operator/correlationlength_model.hh:45-66).
+    The HIP library assembles the stencil itself from (lattice, kappa_sq)."""
+
+    m_lowrank = 0
+
+    def __init__(self, lattice, kappa_sq):
+        self.lattice = lattice
+        self.kappa_sq = float(kappa_sq)
+
+    def get_ndof(self):
+        return self.lattice.Nvertex
+
+    def stencil(self):
+        """(diagonal, off-diagonals per direction) in the reference's arithmetic."""
+        cv = 1.0
+        hinv2 = []
+        for v in self.lattice.n:
+            h = 1.0 / v
+            hinv2.append(1.0 / (h * h))
+            cv *= h
+        diag = cv * self.kappa_sq
+        for d in range(self.lattice.dim):
+            diag += 2.0 * cv * hinv2[d]
+        return diag, [-cv * h2 for h2 in hinv2]
+
+    def get_sparse(self):
+        L = self.lattice
+        N = L.Nvertex
+        idx = L.vertex_euclidean(np.arange(N))
+        diag, off = self.stencil()
+        rows, cols, vals = [np.arange(N)], [np.arange(N)], [np.full(N, diag)]
+        for d in range(L.dim):
+            for s in (-1, 1):
+                j = idx.copy()
+                j[:, d] += s
+                ok = (j[:, d] >= 1) & (j[:, d] <= L.n[d] - 1)
+                rows.append(np.arange(N)[ok])
+                cols.append(L.vertex_index(j[ok]))
+                vals.append(np.full(ok.sum(), off[d]))
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+
+def V_sphere(radius, dim):
+    """Volume of the d-ball (src/linear_operator/measured_operator.cc:52-66)."""
+    if dim == 0:
+        return 1.0
+    if dim == 1:
+        return 2.0 * radius
+    return 2.0 * math.pi / dim * radius * radius * V_sphere(radius, dim - 2)
+
+
+def measurement_vector(lattice, x0, radius):
+    """Dual-space ball average w_j = int f(x) phi_j(x) dx with top-hat f of
+    integral one over the ball of `radius` around x0, 2^d-point Gauss-Legendre
+    per cell (src/linear_operator/measured_operator.cc:69-170); radius < 1e-12
+    selects the nearest vertex (:74-90). Returns (vertex indices, values)."""
+    L = lattice
+    dim = L.dim
+    x0 = np.asarray(x0, dtype=np.float64)
+    if radius < 1e-12:
+        X = L.vertex_coordinates(np.arange(L.Nvertex))
+        dist = np.sqrt(((X - x0) ** 2).sum(axis=1))
+        return np.array([int(np.argmin(dist))]), np.array([1.0])
+    h = np.array([1.0 / v for v in L.n])
+    cv = L.cell_volume()
+    norm = 1.0 / V_sphere(radius, dim)
+    # corners / quadrature points: Cartesian products with the first component slowest
+    bits = np.array([[(c >> (dim - 1 - d)) & 1 for d in range(dim)] for c in range(1 << dim)])
+    p1 = np.array([0.5 * (-1.0 / math.sqrt(3.0) + 1.0), 0.5 * (1.0 / math.sqrt(3.0) + 1.0)])
+    qp = p1[bits]  # (nq, dim)
+    qw = 0.5 ** dim
+    # candidate cells: a bounding box of the ball
+    lo = [max(0, int(math.floor((x0[d] - radius) / h[d])) - 1) for d in range(dim)]
+    hi = [min(L.n[d] - 1, int(math.ceil((x0[d] + radius) / h[d])) + 1) for d in range(dim)]
+    axes = [np.arange(lo[d], hi[d] + 1) for d in range(dim)]
+    grids = np.meshgrid(*axes[::-1], indexing="ij")[::-1]
+    cells = np.stack([g.reshape(-1) for g in grids], axis=1)  # x fastest = ascending linear cell index
+    corners = h * (cells[:, None, :] + bits[None, :, :])
+    cdist = np.sqrt(((corners - x0) ** 2).sum(axis=2))
+    overlap = (cdist < radius).any(axis=1)
+    cmin, cmax = corners.min(axis=1), corners.max(axis=1)
+    overlap |= np.all((cmin <= x0) & (x0 <= cmax), axis=1)
+    cells = cells[overlap]
+    xq = h * (qp[None, :, :] + cells[:, None, :])  # (ncell, nq, dim)
+    inside = np.sqrt(((xq - x0) ** 2).sum(axis=2)) / radius < 1.0
+    rows, vals = [], []
+    for a in range(1 << dim):
+        vert = cells + bits[a]
+        ok = np.all((vert >= 1) & (vert <= np.array(L.n) - 1), axis=1)
+        phihat = np.prod(np.where(bits[a] == 0, 1.0 - qp, qp), axis=1)  # (nq,)
+        contrib = np.where(inside, phihat[None, :] * qw * cv * norm, 0.0)
+        # sum over quadrature points in the reference's order
+        local = np.zeros(len(cells))
+        for q in range(contrib.shape[1]):
+            local = local + contrib[:, q]
+        rows.append(np.where(ok, L.vertex_index(np.where(ok[:, None], vert, 1)), -1))
+        vals.append(local)
+    # accumulate cell by cell, corner by corner (reference order)
+    rows = np.stack(rows, axis=1).reshape(-1)
+    vals = np.stack(vals, axis=1).reshape(-1)
+    keep = rows >= 0
+    rows, vals = rows[keep], vals[keep]
+    uniq, inv = np.unique(rows, return_inverse=True)
+    acc = np.zeros(len(uniq))
+    np.add.at(acc, inv, vals)
+    return uniq, acc
+
+
+class MeasuredOperator:
+    """Posterior precision Q = A + B Sigma^{-1} B^T with B the ball-average
+    measurement matrix and Sigma = variance_scaling * variance
+    (src/linear_operator/measured_operator.cc:9-49)."""
+
+    def __init__(self, base, centres, radius, variance, variance_scaling=1.0):
+        self.base = base
+        self.lattice = base.lattice
+        self.kappa_sq = base.kappa_sq
+        centres = np.asarray(centres, dtype=np.float64).reshape(-1, self.lattice.dim)
+        self.centres = centres
+        self.radius = float(radius)
+        m = len(centres)
+        self.m_lowrank = m
+        self.Sigma = variance_scaling * np.asarray(variance, dtype=np.float64).reshape(m)
+        rows, cols, vals = [], [], []
+        for k in range(m):
+            r, v = measurement_vector(self.lattice, centres[k], radius)
+            rows.append(r)
+            cols.append(np.full(len(r), k))
+            vals.append(v)
+        N = self.lattice.Nvertex
+        if m:
+            self.B = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                                   shape=(N, m))
+        else:
+            self.B = sp.csr_matrix((N, 0))
+        self.B.sort_indices()
+
+    def get_ndof(self):
+        return self.lattice.Nvertex
+
+    def get_sparse(self):
+        return self.base.get_sparse()
+
+    def get_B(self):
+        return self.B
+
+    def get_Sigma(self):
+        return self.Sigma
+
+    def apply(self, x):
+        A = self.get_sparse()
+        return A @ x + self.B @ ((self.B.T @ x) / self.Sigma)
+
+
+def seeded_ball_measurements(lattice, m, radius, variance, seed, low=0.1, high=0.9):
+    """m ball measurements with centres ~ U[low, high]^d and observed values
+    y ~ N(0, 1) from numpy's PCG64 with the given seed."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(low, high, size=(m, lattice.dim))
+    y = rng.standard_normal(m)
+    var = np.full(m, float(variance))
+    return centres, y, var

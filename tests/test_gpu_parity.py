@@ -1,0 +1,177 @@
+"""Parity of every kernel family of the HIP path with the CPU oracle (FP64).
+
+The oracle runs in its multicolour ordering, the order the kernels use, so
+single sweeps, transfers and coarse solves agree to rounding; noise is
+injected through the library's component entry points.
+"""
+import numpy as np
+import pytest
+
+import multigridmc_amd as mg
+from gpu_common import make_problem, normals_reference, oracle_mg, params
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dim, n, nmeas, nlevel
+    (2, (16, 16), 0, 3),
+    (2, (16, 32), 3, 3),
+    (3, (8, 8, 8), 0, 2),
+    (3, (16, 16, 16), 3, 3),
+]
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_level_stencils_match_galerkin(case):
+    dim, n, nmeas, nlevel = case
+    op, ora = make_problem(dim, n, nmeas)
+    p = params(nlevel)
+    s = mg.MultigridMCSampler(op, p, nchains=1)
+    omg = oracle_mg(ora, p)
+    for level in range(nlevel):
+        st, ncol = s.level_stencil(level)
+        A = omg.level_csr(level, 0).toarray()
+        lat = mg.Lattice(*[v >> level for v in n])
+        N = lat.Nvertex
+        idx = lat.vertex_euclidean(np.arange(N))
+        deep = np.where(np.all((idx > 1) & (idx < np.array(lat.n) - 1), axis=1))[0]
+        r = deep[len(deep) // 2]
+        for o in np.ndindex(*([3] * dim)):
+            off = np.array(o) - 1
+            j = lat.vertex_index(idx[r] + off)
+            flat = sum((off[d] + 1) * 3 ** d for d in range(dim))
+            assert abs(st[flat] - A[r, j]) <= 1e-14 * np.abs(A[r]).max()
+        assert ncol == (2 if level == 0 else 2 ** dim)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("omega", [1.0, 0.8])
+def test_sweep_parity(case, omega):
+    """One SOR sweep (+SMW) per level and direction, deterministic and with
+    injected Gibbs noise, against the oracle's multicolour SORSmoother /
+    SORSampler (src/smoother/sor_smoother.cc:41-77, src/sampler/sor_sampler.cc:37-59)."""
+    dim, n, nmeas, nlevel = case
+    op, ora = make_problem(dim, n, nmeas)
+    p = params(nlevel, omega=omega)
+    K = 2
+    s = mg.MultigridMCSampler(op, p, nchains=K)
+    omg = oracle_mg(ora, p)
+    rng = np.random.default_rng(1)
+    for level in range(nlevel):
+        N = s.ndof(level)
+        b, x = rng.standard_normal((K, N)), rng.standard_normal((K, N))
+        xi, xl = rng.standard_normal((K, N)), rng.standard_normal((K, max(nmeas, 1)))
+        for fw in (True, False):
+            got = s.sweep(level, fw, False, b, x)
+            for c in range(K):
+                ref = omg.sor_smoother(level, fw, b[c], x[c])
+                assert _rel(got[c], ref) < 1e-12, (level, fw, c)
+            got = s.sweep(level, fw, True, b, x, xi=xi, xi_lr=xl[:, :nmeas] if nmeas else None)
+            for c in range(K):
+                ref = omg.sor_sampler_noise(level, fw, b[c], x[c], xi[c], xl[c, :nmeas] if nmeas else None)
+                assert _rel(got[c], ref) < 1e-12, (level, fw, c)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_transfer_parity(case):
+    """f_c = R(f - Q x) and x += alpha P x_c (multigridmc_sampler.cc:125-133)."""
+    dim, n, nmeas, nlevel = case
+    op, ora = make_problem(dim, n, nmeas)
+    p = params(nlevel)
+    K = 2
+    s = mg.MultigridMCSampler(op, p, nchains=K)
+    omg = oracle_mg(ora, p)
+    rng = np.random.default_rng(2)
+    for level in range(nlevel - 1):
+        N, Nc = s.ndof(level), s.ndof(level + 1)
+        f, x, xc = rng.standard_normal((K, N)), rng.standard_normal((K, N)), rng.standard_normal((K, Nc))
+        got = s.residual_restrict(level, f, x)
+        got_p = s.prolongate_add(level, 0.7, xc, x)
+        for c in range(K):
+            ref = omg.restrict(level, f[c] - omg.level_apply(level, x[c]))
+            assert _rel(got[c], ref) < 1e-12
+            ref_p = omg.prolongate_add(level, 0.7, xc[c], x[c])
+            assert _rel(got_p[c], ref_p) < 1e-13
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_coarse_parity(case):
+    """Coarsest level: Cholesky sample with injected xi and the deterministic
+    Q_c^{-1} b (cholesky_sampler.hh:52-66, cholesky_solver.cc:30-40)."""
+    dim, n, nmeas, nlevel = case
+    op, ora = make_problem(dim, n, nmeas)
+    p = params(nlevel)
+    K = 3
+    s = mg.MultigridMCSampler(op, p, nchains=K)
+    omg = oracle_mg(ora, p)
+    Nc = s.ndof(nlevel - 1)
+    rng = np.random.default_rng(3)
+    f, xi = rng.standard_normal((K, Nc)), rng.standard_normal((K, Nc))
+    got = s.coarse(f, xi_mode=2, xi=xi)
+    got_d = s.coarse(f, xi_mode=0)
+    for c in range(K):
+        assert _rel(got[c], omg.coarse_sample_noise(f[c], xi[c])) < 1e-11
+        assert _rel(got_d[c], omg.coarse_solve(f[c])) < 1e-11
+
+
+def test_apply_operator():
+    op, ora = make_problem(3, (16, 16, 16), 3)
+    s = mg.MultigridMCSampler(op, params(3), nchains=2)
+    x = np.random.default_rng(4).standard_normal((2, op.lattice.Nvertex))
+    y = s.apply_operator(x)
+    for c in range(2):
+        assert _rel(y[c], ora.apply(x[c])) < 1e-13
+
+
+def test_normals_match_philox():
+    """The Gaussians the sweeps draw equal Philox4x32-10 + Box-Muller computed
+    independently in NumPy (tolerance covers the fast log/sin/cos), and pass the
+    reference's normal-CDF test (test_random.hh:82-109)."""
+    lat = mg.Lattice(64, 64, 64)
+    op = mg.ShiftedLaplaceFDOperator(lat, 10.0)
+    seed = 0x1234567890ABCDEF
+    s = mg.MultigridMCSampler(op, params(4), nchains=4, seed=seed)
+    s.set_counter(77)
+    z = s.normals(5)
+    ref = normals_reference(lat, 4, 5, 77, seed)
+    # the integer stream is exact; the FP32 v_log/v_sin/v_cos Box-Muller differs
+    # from the FP64 NumPy evaluation by a few ulp of the result
+    err = np.abs(z - ref)
+    assert err.max() < 1e-4 and err.mean() < 1e-6
+    flat = z.reshape(-1)
+    import math
+    h, edges = np.histogram(flat, bins=64, range=(-10, 10))
+    cdf = np.cumsum(h) / flat.size
+    true = np.array([0.5 * (1 + math.erf(v / math.sqrt(2))) for v in edges[1:]])
+    assert np.abs(cdf - true).max() < 2e-3
+    assert abs(flat.mean()) < 5e-3 and abs(flat.var() - 1) < 5e-3
+
+
+def test_csr_operator_input_matches_fd():
+    """A supplied as the reference's CSR matrix is recognised as the same stencil."""
+    import ctypes as C
+    from multigridmc_amd import _lib
+    op, ora = make_problem(3, (16, 16, 16), 0)
+    s_fd = mg.MultigridMCSampler(op, params(3))
+    A = op.get_sparse().tocsr()
+    A.sort_indices()
+    prob, keep = mg.sampler._problem(op)
+    rp, ci, v = _lib.i64(A.indptr), _lib.i64(A.indices), _lib.f64(A.data)
+    prob.A_rowptr, prob.A_col, prob.A_val = _lib.ptr(rp), _lib.ptr(ci), _lib.ptr(v)
+    par = _lib.Params(nlevel=3, smoother=1, npresmooth=1, npostsmooth=1, ncoarsesmooth=1, omega=1.0, cycle=1,
+                      coarse_scaling=1.0, coarse_solver=0, precision=0, nchains=1, seed=1, stream=None)
+    h = C.c_void_p()
+    _lib.check(_lib.lib().mgmc_create(C.byref(prob), C.byref(par), C.byref(h)))
+    try:
+        for level in range(3):
+            st = np.zeros(27)
+            nc = C.c_int()
+            _lib.check(_lib.lib().mgmc_level_stencil(h, level, _lib.ptr(st), C.byref(nc)))
+            assert np.allclose(st, s_fd.level_stencil(level)[0], rtol=1e-14, atol=0)
+    finally:
+        _lib.lib().mgmc_destroy(h)

@@ -1,0 +1,149 @@
+"""End-to-end GPU checks: the driver_mg solve path and MGMC sampling statistics."""
+import numpy as np
+import pytest
+
+import multigridmc_amd as mg
+from gpu_common import make_problem, oracle_mg, params
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dim,n,nmeas,nlevel", [(2, (64, 64), 0, 4), (2, (64, 64), 4, 4), (3, (32, 32, 32), 6, 4)])
+@pytest.mark.parametrize("cycle", [1, 2])
+def test_mg_solve_fp64(dim, n, nmeas, nlevel, cycle):
+    """LoopSolver + MG (driver_mg path, src/driver_mg.cc:154-173): FP64 solution
+    within 1e-10 relative of the exact solution, per-iteration residuals equal to
+    the oracle's multicolour twin, and convergence of the lexicographic
+    reference ordering to the same solution."""
+    op, ora = make_problem(dim, n, nmeas, radius=0.1, var=1e-4)
+    p = params(nlevel, cycle=cycle)
+    pre = mg.MultigridPreconditioner(op, p)
+    solver = mg.LoopSolver(op, pre, rtol=1e-11, atol=1e-9, maxiter=60)
+    b = np.random.default_rng(1482817).standard_normal(op.lattice.Nvertex)
+    x = solver.apply(b)
+    omg = oracle_mg(ora, p)
+    x_ref, hist_ref, it_ref = omg.loop_solve(b, rtol=1e-11, atol=1e-9, maxiter=60)
+    assert solver.niter == it_ref
+    h = solver.history
+    sig = hist_ref / hist_ref[0] > 1e-8
+    assert np.allclose(h[sig], hist_ref[sig], rtol=1e-8)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-10
+    lex = oracle_mg(ora, p, ordering=po.LEX)
+    x_lex, hist_lex, _ = lex.loop_solve(b, rtol=1e-11, atol=1e-9, maxiter=60)
+    assert np.linalg.norm(x - x_lex) / np.linalg.norm(x_lex) < 1e-10
+    rho = h[1:] / h[:-1]
+    assert rho.max() < 0.3
+
+
+def test_mg_precondition_matches_oracle():
+    op, ora = make_problem(3, (16, 16, 16), 3)
+    p = params(3)
+    pre = mg.MultigridPreconditioner(op, p)
+    b = np.random.default_rng(5).standard_normal(op.lattice.Nvertex)
+    got = pre.apply(b)
+    ref = oracle_mg(ora, p).precondition(b)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("smoother", ["SSOR", "SOR"])
+def test_mg_solve_fp32_with_fp64_refinement(smoother):
+    """FP32 cycle, FP64 residual and update: still converges to 1e-10."""
+    op, ora = make_problem(3, (32, 32, 32), 6, radius=0.1, var=1e-4)
+    p = params(4, smoother=smoother)
+    pre = mg.MultigridPreconditioner(op, p, precision="fp32")
+    solver = mg.LoopSolver(op, pre, rtol=1e-11, atol=1e-9, maxiter=80)
+    b = np.random.default_rng(7).standard_normal(op.lattice.Nvertex)
+    x = solver.apply(b)
+    x_ref, _, _ = oracle_mg(ora, params(4, smoother=smoother)).loop_solve(b, rtol=1e-12, atol=1e-12, maxiter=80)
+    assert solver.niter > 0
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-10
+
+
+def _posterior_moments(ora, f, probes):
+    Q = ora.dense_precision()
+    cov = np.linalg.inv(Q)
+    return cov @ f, cov[np.ix_(probes, probes)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("smoother,cycle", [("SSOR", 1), ("SOR", 2)])
+def test_mgmc_posterior_moments(precision, smoother, cycle):
+    """Batched MGMC chains sample N(Q^-1 f, Q^-1): means and covariances at
+    probe vertices agree with the exact posterior within 4 Monte Carlo
+    standard errors (the statistical test of test_sampler.hh:113-153)."""
+    dim, n = 2, (16, 16)
+    op, ora = make_problem(dim, n, 2, radius=0.15, var=1e-3)
+    N = op.lattice.Nvertex
+    rng = np.random.default_rng(1342517)
+    mean_exact = rng.uniform(size=N)
+    f = ora.dense_precision() @ mean_exact
+    probes = rng.choice(N, size=12, replace=False)
+    K, nwarm, ns = 256, 20, 1000
+    s = mg.MultigridMCSampler(op, params(3, smoother=smoother, cycle=cycle), nchains=K, precision=precision,
+                              seed=2024)
+    s.fix_rhs(f)
+    s.set_state(np.zeros((K, N)))
+    s.set_probes(probes)
+    s.run(nwarm, record=False)
+    obs = s.run(ns)
+    mu, cov = _posterior_moments(ora, f, probes)
+    tau, _ = mg.tau_int_chains(obs[:, :, 0])
+    neff = K * ns / max(tau, 1.0)
+    m_emp = obs.mean(axis=(0, 1))
+    se_mean = np.sqrt(np.diag(cov) / neff)
+    assert np.all(np.abs(m_emp - mu[probes]) < 4 * se_mean + 1e-6)
+    d = obs.reshape(-1, len(probes)) - m_emp
+    cov_emp = d.T @ d / d.shape[0]
+    se_cov = np.sqrt((np.outer(np.diag(cov), np.diag(cov)) + cov**2) / neff)
+    assert np.all(np.abs(cov_emp - cov) < 4 * se_cov + 1e-6)
+    assert tau < 3.0
+
+
+def test_graph_replay_matches_eager():
+    """A captured hipGraph of the cycle replays exactly the eager launches."""
+    op, _ = make_problem(3, (16, 16, 16), 3)
+    N = op.lattice.Nvertex
+    f = np.random.default_rng(8).standard_normal(N)
+    res = []
+    for use_graph in (0, 1):
+        s = mg.MultigridMCSampler(op, params(3), nchains=4, precision="fp32", seed=99)
+        s.fix_rhs(f)
+        s.set_state(np.zeros((4, N)))
+        s.set_probes(np.arange(0, N, 97))
+        s.set_counter(5)
+        obs = s.run(7, use_graph=bool(use_graph))
+        res.append((obs, s.get_state()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_sample_api_matches_resident_run():
+    """Sampler::apply(f, x) (copy in/out) equals the resident loop."""
+    op, _ = make_problem(2, (32, 32), 2)
+    N = op.lattice.Nvertex
+    f = np.random.default_rng(9).standard_normal(N)
+    a = mg.MultigridMCSampler(op, params(4), nchains=2, seed=3)
+    b = mg.MultigridMCSampler(op, params(4), nchains=2, seed=3)
+    x = np.zeros((2, N))
+    for _ in range(3):
+        a.apply(f, x)
+    b.fix_rhs(f)
+    b.set_state(np.zeros((2, N)))
+    b.run(3, record=False, use_graph=False)
+    assert np.array_equal(x, b.get_state())
+
+
+def test_chains_differ_and_seed_reproduces():
+    op, _ = make_problem(2, (32, 32), 0)
+    N = op.lattice.Nvertex
+    f = np.zeros(N)
+    outs = []
+    for seed in (1, 1, 2):
+        s = mg.MultigridMCSampler(op, params(4), nchains=2, seed=seed)
+        x = np.zeros((2, N))
+        s.apply(f, x)
+        outs.append(x)
+    assert np.array_equal(outs[0], outs[1])
+    assert not np.allclose(outs[0], outs[2])
+    assert not np.allclose(outs[0][0], outs[0][1])
