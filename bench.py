@@ -42,15 +42,15 @@ def tau_max(obs):
     return float(max(taus)), float(np.median(taus))
 
 
-def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed):
+def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed, use_graph=True):
     import multigridmc_amd as mg
     s = mg.MultigridMCSampler(op, params, nchains=nchains, precision=precision, seed=seed)
     s.fix_rhs(f)
     s.set_state(np.zeros((nchains, op.lattice.Nvertex)))
     s.set_probes(probes)
-    s.run(warmup, record=False)
+    s.run(warmup, record=False, use_graph=use_graph)
     t0 = time.perf_counter()
-    obs = s.run(steps)  # synchronises the handle's stream before returning
+    obs = s.run(steps, use_graph=use_graph)  # synchronises the handle's stream before returning
     t = time.perf_counter() - t0
     return s, obs, t
 
@@ -100,7 +100,21 @@ def main():
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (chains would shard across "
@@ -118,7 +132,8 @@ def main():
                                     coarse_solver="Cholesky")
 
     # batched FP32 (the metric's form)
-    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1)
+    g = not args.no_graph
+    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1, g)
     tau32, tau32_med = tau_max(obs32)
     ess32 = K * args.steps / (t32 * tau32)
 
@@ -131,13 +146,13 @@ def main():
     bytes_per_launch = 2 * w * N * K  # read x (all), read f + write x (one colour)
     achieved = bytes_per_launch / (ms_sweep * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "sweep_l0_fp32_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "sweep_l0_fp32_128cube_k64_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
     # plain single-chain FP64
-    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2)
+    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2, g)
     tau64, tau64_med = tau_max(obs64)
     ess64 = args.fp64_steps / (t64 * tau64)
 
