@@ -42,9 +42,11 @@ def tau_max(obs):
     return float(max(taus)), float(np.median(taus))
 
 
-def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed, use_graph=True):
+def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed, use_graph=True, tuning=None):
     import multigridmc_amd as mg
     s = mg.MultigridMCSampler(op, params, nchains=nchains, precision=precision, seed=seed)
+    for k, v in (tuning or {}).items():
+        s.set_tuning(k, v)
     s.fix_rhs(f)
     s.set_state(np.zeros((nchains, op.lattice.Nvertex)))
     s.set_probes(probes)
@@ -101,6 +103,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
+    ap.add_argument("--colour-phases", action="store_true",
+                    help="sweep level 0 with one launch per colour instead of the single-pass kernel")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -133,26 +137,33 @@ def main():
 
     # batched FP32 (the metric's form)
     g = not args.no_graph
-    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1, g)
+    tuning = {"fused_sweep": 0} if args.colour_phases else {}
+    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1, g, tuning)
     tau32, tau32_med = tau_max(obs32)
     ess32 = K * args.steps / (t32 * tau32)
 
-    # dominant kernel: level-0 colour-phase sweep, timed with HIP events on the handle's stream
+    # dominant kernel: the level-0 sweep, timed with HIP events on the handle's stream
     s32.profile(1)
     s32.run(10, record=False)
     ms_sweep, nlaunch = s32.profile_result()
     s32.profile(0)
     w = 4
-    bytes_per_launch = 2 * w * N * K  # read x (all), read f + write x (one colour)
+    if args.colour_phases:
+        kernel = "k_sweep<float,3,star,philox> (level-0 red-black colour phase)"
+        bytes_per_launch = 2 * w * N * K  # read x (all), read f + write x (one colour)
+    else:
+        kernel = "k_sweep_rb3<float,philox> (level-0 red-black sweep, one pass)"
+        bytes_per_launch = 3 * w * N * K  # read x, read f, write x
     achieved = bytes_per_launch / (ms_sweep * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "sweep_l0_fp32_128cube_k64_traffic.json")
+    tf = os.path.join(ROOT, "profiles", ("sweep_l0_fp32_128cube_k64_traffic.json" if args.colour_phases
+                                         else "sweep_rb3_l0_fp32_128cube_k64_traffic.json"))
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
     # plain single-chain FP64
-    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2, g)
+    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2, g, tuning)
     tau64, tau64_med = tau_max(obs64)
     ess64 = args.fp64_steps / (t64 * tau64)
 
@@ -185,7 +196,7 @@ def main():
         "speedup_vs_fp64_single": ess32 / ess64,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep<float,3,star,philox> (level-0 red-black colour phase)",
+                     "kernel": kernel,
                      "avg_launch_ms": ms_sweep, "launches_timed": nlaunch,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,

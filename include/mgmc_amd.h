@@ -156,8 +156,16 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out);
  * offsets -1..1 per direction) and the colour count (2 = red-black). */
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);
 
-/* Profiling aid: with enable = 1 every level-0 sweep colour-phase launch is
- * bracketed by HIP events on the handle's stream (runs then launch eagerly).
+/* Launch-shape tuning (no reference counterpart): key "fused_sweep" (0/1:
+ * sweep 3D star levels with the single-pass red-black kernel), "rb_threads"
+ * (threads per workgroup of that kernel, 64..1024), "rb_blocks" (workgroups
+ * it aims for; more z-segments when fewer tiles x chains exist). Results do
+ * not depend on these settings beyond rounding. */
+int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
+
+/* Profiling aid: with enable = 1 every level-0 sweep launch (one fused
+ * red-black sweep, or one colour phase) is bracketed by HIP events on the
+ * handle's stream (runs then launch eagerly).
  * mgmc_profile_result returns the mean duration of the launches recorded
  * since the last call and their number. No reference counterpart. */
 int mgmc_profile(mgmc_handle* h, int enable);

@@ -79,6 +79,7 @@ _SIGS = {
     "mgmc_coarse": [_vp, _vp, _vp, C.c_int, _vp],
     "mgmc_normals": [_vp, C.c_uint32, _vp],
     "mgmc_level_stencil": [_vp, C.c_int, _vp, C.POINTER(C.c_int)],
+    "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],
     "mgmc_profile": [_vp, C.c_int],
     "mgmc_profile_result": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }

@@ -41,6 +41,79 @@ __device__ __forceinline__ void store8(T* __restrict__ p, const T* v) {
   }
 }
 
+// Star-stencil update of the four vertices of parity IPAR in an 8-vertex
+// group: xc holds the group, `left`/`right` its outer neighbours, ym/yp/zm/zp
+// the same group in the neighbouring rows/planes, fb the right-hand side.
+// IPAR is a template parameter so every index below is a compile-time
+// constant and the groups stay in registers.
+template <int IPAR, typename T, int NOISE>
+__device__ __forceinline__ void star_update4(T* xc, T left, T right, const T* ym, const T* yp, const T* zm,
+                                             const T* zp, const T* fb, const float* xi, const T* extg, int i0, int n0,
+                                             T cx, T cy, T cz, T D, T om_d, T ns) {
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int q = 2 * l + IPAR;
+    const T xl = q > 0 ? xc[q - 1] : left;
+    const T xr = q < 7 ? xc[q + 1] : right;
+    T acc = cy * ym[q] + cx * xl + D * xc[q] + cx * xr + cy * yp[q];
+    if (zm) acc = cz * zm[q] + acc + cz * zp[q];
+    const int i = i0 + q;
+    if (i >= 1 && i <= n0 - 1) {
+      T bb = fb[q];
+      if (NOISE == NOISE_PHILOX) bb += ns * (T)xi[l];
+      if (NOISE == NOISE_EXTERNAL) bb += ns * extg[q];
+      xc[q] += om_d * (bb - acc);
+    }
+  }
+}
+
+// Box-stencil (9/27-point) update of the four vertices of parity IPAR in an
+// 8-vertex group; xc[q + 1] = x(i0 + q), xc[0] and xc[9] the outer
+// neighbours, xr the group's address (neighbour rows are loaded from it).
+template <int IPAR, typename T, int DIM, int NOISE>
+__device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g, const Stencil& st, const T* fb,
+                                            const float* xi, const T* extg, int i0) {
+  T acc[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) acc[l] = (T)0;
+  const long long pl = (long long)g.px * g.py;
+#pragma unroll
+  for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      T w[10];
+      if (dz == 0 && dy == 0) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) w[q] = xc[q];
+      } else {
+        const T* p = xr + dz * pl + dy * g.px;
+        load8<T>(p, w + 1);
+        w[0] = p[-1];
+        w[9] = p[8];
+      }
+      const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
+      const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
+      const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int q = 2 * l + IPAR;
+        acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
+      }
+    }
+  }
+  const T om_d = (T)st.inv_diag_omega;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int q = 2 * l + IPAR;
+    const int i = i0 + q;
+    if (i < 1 || i > g.n0 - 1) continue;
+    T b = fb[q];
+    if (NOISE == NOISE_PHILOX) b += (T)st.noise_scale * (T)xi[l];
+    if (NOISE == NOISE_EXTERNAL) b += (T)st.noise_scale * extg[q];
+    xc[q + 1] += om_d * (b - acc[l]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One colour phase of an SOR sweep (SORSmoother::apply_sparse,
 // src/smoother/sor_smoother.cc:56-77, restricted to one colour), with the
@@ -95,70 +168,32 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
     normals4(c0, (uint32_t)chain, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
   }
 
-  T acc[4];
   if (STAR) {
-    T ym[8], yp[8];
+    T ym[8], yp[8], zm[8], zp[8];
     load8<T>(xr - g.px, ym);
     load8<T>(xr + g.px, yp);
-    const T cx = (T)st.c[9 + 3 + 0], cy = (T)st.c[9 + 0 + 1];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      int q = 2 * l + ipar;
-      acc[l] = cy * ym[q] + cx * xc[q] + (T)st.diag * xc[q + 1] + cx * xc[q + 2] + cy * yp[q];
-    }
     if (DIM == 3) {
-      T zm[8], zp[8];
       const long long pl = (long long)g.px * g.py;
       load8<T>(xr - pl, zm);
       load8<T>(xr + pl, zp);
-      const T cz = (T)st.c[0 + 3 + 1];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        int q = 2 * l + ipar;
-        acc[l] = cz * zm[q] + acc[l] + cz * zp[q];
-      }
     }
-  } else {
-#pragma unroll
-    for (int l = 0; l < 4; ++l) acc[l] = (T)0;
-    const long long pl = (long long)g.px * g.py;
-#pragma unroll
-    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
-#pragma unroll
-      for (int dy = -1; dy <= 1; ++dy) {
-        T w[10];
-        if (dz == 0 && dy == 0) {
-#pragma unroll
-          for (int q = 0; q < 10; ++q) w[q] = xc[q];
-        } else {
-          const T* p = xr + dz * pl + dy * g.px;
-          load8<T>(p, w + 1);
-          w[0] = p[-1];
-          w[9] = p[8];
-        }
-        const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
-        const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
-        const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-          int q = 2 * l + ipar;
-          acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
-        }
-      }
-    }
+    const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4];
+    const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
+    const T* zmp = DIM == 3 ? zm : nullptr;
+    if (ipar == 0)
+      star_update4<0, T, NOISE>(xc + 1, xc[0], xc[9], ym, yp, zmp, zp, fb, xi, eg, 8 * gx, g.n0, cx, cy, cz,
+                                (T)st.diag, (T)st.inv_diag_omega, (T)st.noise_scale);
+    else
+      star_update4<1, T, NOISE>(xc + 1, xc[0], xc[9], ym, yp, zmp, zp, fb, xi, eg, 8 * gx, g.n0, cx, cy, cz,
+                                (T)st.diag, (T)st.inv_diag_omega, (T)st.noise_scale);
+    store8<T>(xr, xc + 1);
+    return;
   }
-
-  const T om_d = (T)st.inv_diag_omega;
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    int q = 2 * l + ipar;
-    int i = 8 * gx + q;
-    if (i < 1 || i > g.n0 - 1) continue;
-    T b = fb[q];
-    if (NOISE == NOISE_PHILOX) b += (T)st.noise_scale * (T)xi[l];
-    if (NOISE == NOISE_EXTERNAL) b += (T)st.noise_scale * ext[base + q];
-    xc[q + 1] += om_d * (b - acc[l]);
-  }
+  const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
+  if (ipar == 0)
+    box_update4<0, T, DIM, NOISE>(xc, xr, g, st, fb, xi, eg, 8 * gx);
+  else
+    box_update4<1, T, DIM, NOISE>(xc, xr, g, st, fb, xi, eg, 8 * gx);
   store8<T>(xr, xc + 1);
 }
 
@@ -330,6 +365,12 @@ __global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* 
 }
 
 template <typename T>
+__global__ void __launch_bounds__(256) k_copy(T* __restrict__ dst, const T* __restrict__ src, long long n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+template <typename T>
 __global__ void __launch_bounds__(256) k_fill(T* __restrict__ p, long long n, T v) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -399,6 +440,135 @@ __global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __re
       for (int j = r; j < Nc; ++j) s += l[j] * xs[j];
     }
     x[c * cs + pidx[r]] = (T)s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One whole red-black SOR/Gibbs sweep of a 3D star-stencil level in a single
+// pass over HBM. A workgroup owns full x-rows (one thread per aligned group
+// of eight vertices) for TY owned y-rows plus two halo rows on each side, and
+// marches through its z-segment [ks, ke): at step k it applies the first
+// colour to plane k and the second colour to plane k-1, then stores plane
+// k-1. The first colour is also applied on the halo ring (rows j0-1, j0+TY,
+// planes ks-1, ke) and discarded: those values depend only on old values and
+// on counter-based noise keyed by the vertex, so they equal the values the
+// owning workgroup computes. The sweep is out of place (x -> xout) so that
+// every workgroup reads old values in its halo; every interior vertex of xout
+// is written exactly once. Result = the two colour-phase launches of k_sweep
+// (same Philox slots, same update formula).
+// Per step: the loads of plane k+1 and f(k) are issued first and the Philox
+// normals of both colours are computed while they are in flight; LDS holds
+// plane k before the first colour (TY+4 rows) and plane k-1 after it (TY+2
+// rows), each double-buffered so one barrier per step suffices.
+// ---------------------------------------------------------------------------
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
+                                                    const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
+                                                    Stencil st, int first, int TY, int SZ, int ntile, int nseg,
+                                                    NoiseKey nk) {
+  MGMC_DYNAMIC_SHARED(char, lds_raw);
+  const size_t asz = (size_t)(TY + 4) * g.px, bsz = (size_t)(TY + 2) * g.px;
+  T* A0 = reinterpret_cast<T*>(lds_raw);
+  T* B0 = A0 + 2 * asz;
+  const int ngx = g.px >> 3;
+  const int tid = threadIdx.x;
+  const int r = tid / ngx, gx = tid % ngx;
+  int b = blockIdx.x;
+  const int tile = b % ntile;
+  b /= ntile;
+  const int seg = b % nseg;
+  const int chain = b / nseg;
+  const int j0 = 1 + tile * TY;
+  const int j = j0 - 2 + r;
+  const bool active = r < TY + 4;
+  const bool has_mem = active && j >= 0 && j <= g.n1;
+  const bool ring = active && j >= j0 - 1 && j <= j0 + TY && j >= 1 && j <= g.n1 - 1;
+  const bool owned = active && j >= j0 && j <= j0 + TY - 1 && j <= g.n1 - 1;
+  const int ks = 1 + seg * SZ;
+  const int ke = ks + SZ < g.n2 ? ks + SZ : g.n2;  // owned interior planes [ks, ke)
+  const long long pl = (long long)g.px * g.py;
+  const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+  const uint32_t sample = *nk.sample;
+  const int second = 1 - first;
+
+  T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = (T)0;
+  if (has_mem && ks - 2 >= 0) load8<T>(x + cbase + (ks - 2) * pl, xm);
+  if (has_mem) load8<T>(x + cbase + (ks - 1) * pl, xc);
+
+  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
+  const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
+  const size_t aoff = (size_t)r * g.px + 8 * gx;        // A rows j0-2 .. j0+TY+1
+  const size_t boff = (size_t)(r - 1) * g.px + 8 * gx;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
+
+  for (int k = ks - 1; k <= ke; ++k) {
+    const int kb = k - 1;
+    const bool doA = ring && k >= 1 && k <= g.n2 - 1;
+    const bool doB = owned && kb >= ks && kb <= ke - 1;
+    // loads first: next plane of x, f on plane k
+    if (has_mem && k + 1 <= g.n2) load8<T>(x + cbase + (long long)(k + 1) * pl, xp);
+    else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xp[q] = (T)0;
+    }
+    if (doA) load8<T>(f + cbase + (long long)k * pl, fc);
+    // noise of both colours while the loads are in flight
+    const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
+    float xiA[4] = {0.f, 0.f, 0.f, 0.f}, xiB[4] = {0.f, 0.f, 0.f, 0.f};
+    if (NOISE == NOISE_PHILOX) {
+      if (doA) {
+        uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + gx) << 1) | (uint32_t)iparA;
+        normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
+      }
+      if (doB) {
+        uint32_t c0 = (uint32_t)((((long long)kb * g.py + j) * ngx + gx) << 1) | (uint32_t)iparB;
+        normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+      }
+    }
+    T* A = A0 + (k & 1) * asz;         // plane k before the first colour
+    const T* Bm = B0 + (kb & 1) * bsz;  // plane k-1 after the first colour (written last step)
+    T* Bk = B0 + (k & 1) * bsz;
+    if (active) store8<T>(A + aoff, xc);
+    __syncthreads();
+    // first colour on plane k (owned rows and halo ring)
+    if (doA) {
+      const T* Arow = A + aoff;
+      T ym[8], yp[8];
+      load8<T>(Arow - g.px, ym);
+      load8<T>(Arow + g.px, yp);
+      const T left = gx > 0 ? Arow[-1] : (T)0;
+      const T right = gx < ngx - 1 ? Arow[8] : (T)0;
+      const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)k * pl : nullptr;
+      if (iparA == 0)
+        star_update4<0, T, NOISE>(xc, left, right, ym, yp, xm, xp, fc, xiA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
+      else
+        star_update4<1, T, NOISE>(xc, left, right, ym, yp, xm, xp, fc, xiA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
+    }
+    // second colour on plane k-1 (owned rows, owned planes)
+    if (doB) {
+      const T* Brow = Bm + boff;
+      T ym[8], yp[8];
+      load8<T>(Brow - g.px, ym);
+      load8<T>(Brow + g.px, yp);
+      const T left = gx > 0 ? Brow[-1] : (T)0;
+      const T right = gx < ngx - 1 ? Brow[8] : (T)0;
+      const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)kb * pl : nullptr;
+      if (iparB == 0)
+        star_update4<0, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
+      else
+        star_update4<1, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
+      store8<T>(xout + cbase + (long long)kb * pl, xm);
+    }
+    // plane k after the first colour, for the second colour at the next step
+    if (active && r >= 1 && r <= TY + 2) store8<T>(Bk + boff, xc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      xmm[q] = xm[q];
+      xm[q] = xc[q];
+      xc[q] = xp[q];
+      fm[q] = fc[q];
+    }
   }
 }
 

@@ -119,6 +119,11 @@ struct Common {
   DevBuf<long long> probe_pidx;
   int nprobe = 0;
   uint32_t tag = 0;
+  // tuning: fused red-black sweep on 3D star levels, its threads per block
+  // and the number of workgroups it aims for
+  int tune_fused = 1;
+  int tune_rb_threads = 512;
+  int tune_rb_blocks = 1024;
   // optional HIP-event timing of the level-0 colour-phase launches
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -146,6 +151,7 @@ struct EngineBase {
   virtual void fine_residual_fp64(const double* x_pad64, const double* b_pad64, double* r_pad64) = 0;
   virtual void set_f_from_fp64(const double* pad64) = 0;
   virtual void add_x_to_fp64(double* pad64) = 0;
+  virtual void retune() = 0;
 };
 
 // A level field for all chains with a zeroed guard of GUARD elements on both
@@ -169,24 +175,121 @@ template <typename T>
 struct Engine : EngineBase {
   Common& C;
   std::vector<Field<T>> x, f, r;
+  // second state buffer of levels swept out of place by k_sweep_rb3; cur[l]
+  // says which buffer holds the current state (0 = x, 1 = x2). Between API
+  // calls the state is always in x (normalise()).
+  std::vector<Field<T>> x2;
+  std::vector<int> cur;
   Field<T> ext;  // external noise staging (test hooks)
+
+  T* X(int l) { return cur[l] ? x2[l].p : x[l].p; }
 
   explicit Engine(Common& c) : C(c) {
     x.resize(C.nlevel);
     f.resize(C.nlevel);
     r.resize(C.nlevel);
+    x2.resize(C.nlevel);
+    cur.assign(C.nlevel, 0);
     for (int l = 0; l < C.nlevel; ++l) {
       size_t n = (size_t)C.L[l]->g.cs * C.K;
       x[l].alloc(n, C.stream);
       f[l].alloc(n, C.stream);
       r[l].alloc(n, C.stream);
+      if (fused_ok(l)) x2[l].alloc(n, C.stream);
     }
     HIP_CHECK(hipStreamSynchronize(C.stream));
+  }
+
+  // ---- fused red-black sweep (k_sweep_rb3) configuration ----
+  struct RbShape {
+    int TY, SZ, ntile, nseg, threads;
+    size_t lds;
+  };
+  // rows per tile: threads per workgroup and 64 KB of double-buffered LDS
+  int rb_ty(int l) const {
+    const DevGeom& g = C.L[l]->g;
+    int ngx = g.px >> 3;
+    int by_threads = C.tune_rb_threads / ngx - 4;
+    int by_lds = (int)((65536 / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
+    return std::min(std::min(by_threads, by_lds), g.n1 - 1);
+  }
+  bool fused_ok(int l) const {
+    const Level& Lv = *C.L[l];
+    if (!C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
+    return rb_ty(l) >= std::min(4, Lv.g.n1 - 1);
+  }
+  RbShape rb_shape(int l) const {
+    const DevGeom& g = C.L[l]->g;
+    RbShape s;
+    int ngx = g.px >> 3;
+    s.TY = rb_ty(l);
+    s.ntile = (g.n1 - 1 + s.TY - 1) / s.TY;
+    int nplanes = g.n2 - 1;
+    int want = (C.tune_rb_blocks + s.ntile * C.K - 1) / (s.ntile * C.K);
+    int nseg = std::max(1, std::min(want, std::max(1, nplanes / 8)));
+    s.SZ = (nplanes + nseg - 1) / nseg;
+    s.nseg = (nplanes + s.SZ - 1) / s.SZ;
+    s.threads = ((ngx * (s.TY + 4) + 63) / 64) * 64;
+    s.lds = (size_t)2 * (2 * s.TY + 6) * g.px * sizeof(T);
+    return s;
+  }
+  void fused_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
+    Level& Lv = *C.L[l];
+    RbShape s = rb_shape(l);
+    T* in = X(l);
+    T* out = cur[l] ? x[l].p : x2[l].p;
+    NoiseKey nk = key(tag);
+    dim3 grid((unsigned)(s.ntile * s.nseg * C.K));
+    int first = forward ? 0 : 1;
+    const bool timed = prof_begin(l);
+    if (noise == NOISE_NONE)
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, Lv.g, Lv.st, first,
+             s.TY, s.SZ, s.ntile, s.nseg, nk);
+    else if (noise == NOISE_PHILOX)
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_PHILOX>, in, out, f[l].p, extp, Lv.g, Lv.st,
+             first, s.TY, s.SZ, s.ntile, s.nseg, nk);
+    else
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_EXTERNAL>, in, out, f[l].p, extp, Lv.g, Lv.st,
+             first, s.TY, s.SZ, s.ntile, s.nseg, nk);
+    if (timed) prof_end();
+    cur[l] ^= 1;
+  }
+  void retune() override {
+    for (int l = 0; l < C.nlevel; ++l)
+      if (fused_ok(l) && !x2[l].p) x2[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+  }
+  // bring every level's state back into x (end of a cycle / before host access)
+  void normalise() {
+    for (int l = 0; l < C.nlevel; ++l)
+      if (cur[l]) {
+        long long n = (long long)C.L[l]->g.cs * C.K;
+        launch(nblocks(n), 256, 0, C.stream, k_copy<T>, x[l].p, x2[l].p, n);
+        cur[l] = 0;
+      }
   }
 
   NoiseKey key(uint32_t tag) const { return NoiseKey{C.k0, C.k1, tag, C.ctr.p}; }
 
   void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, k_fill<T>, p, n, v); }
+
+  // HIP events around level-0 sweep launches when profiling
+  bool prof_begin(int l) {
+    if (!(C.prof && l == 0)) return false;
+    if (C.ev_used + 2 > C.ev.size()) {
+      for (int q = 0; q < 64; ++q) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        C.ev.push_back(e);
+      }
+    }
+    HIP_CHECK(hipEventRecord(C.ev[C.ev_used], C.stream));
+    return true;
+  }
+  void prof_end() {
+    HIP_CHECK(hipEventRecord(C.ev[C.ev_used + 1], C.stream));
+    C.ev_used += 2;
+  }
 
   // colour phase launch
   void colour_phase(int l, int colour, int noise, const T* extp, uint32_t tag) {
@@ -205,19 +308,9 @@ struct Engine : EngineBase {
     long long threads = (long long)nrows * C.K * (g.px >> 3);
     dim3 grid(nblocks(threads)), blk(256);
     NoiseKey nk = key(tag);
-    const bool timed = C.prof && l == 0;
-    if (timed) {
-      if (C.ev_used + 2 > C.ev.size()) {
-        for (int q = 0; q < 64; ++q) {
-          hipEvent_t e;
-          HIP_CHECK(hipEventCreate(&e));
-          C.ev.push_back(e);
-        }
-      }
-      HIP_CHECK(hipEventRecord(C.ev[C.ev_used], C.stream));
-    }
+    const bool timed = prof_begin(l);
 #define MGMC_SWEEP(DIM, STAR, NOISE) \
-  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, x[l].p, f[l].p, extp, g, Lv.st, colour, nrows, C.K, nk)
+  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, g, Lv.st, colour, nrows, C.K, nk)
 #define MGMC_SWEEP_N(DIM, STAR)                                 \
   do {                                                          \
     if (noise == NOISE_NONE) MGMC_SWEEP(DIM, STAR, NOISE_NONE); \
@@ -233,10 +326,7 @@ struct Engine : EngineBase {
     }
 #undef MGMC_SWEEP_N
 #undef MGMC_SWEEP
-    if (timed) {
-      HIP_CHECK(hipEventRecord(C.ev[C.ev_used + 1], C.stream));
-      C.ev_used += 2;
-    }
+    if (timed) prof_end();
   }
 
   void lr_dot(int l, const T* xp) {
@@ -266,15 +356,20 @@ struct Engine : EngineBase {
   void dir_sweep(int l, bool forward, int noise, const T* extp, const double* extlr, int sparse_repeats = 1) {
     Level& Lv = *C.L[l];
     uint32_t tag = C.tag++;
-    for (int s = 0; s < sparse_repeats; ++s)
+    for (int s = 0; s < sparse_repeats; ++s) {
+      if (fused_ok(l)) {
+        fused_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
+        continue;
+      }
       for (int q = 0; q < Lv.ncolour; ++q) {
         int colour = forward ? q : Lv.ncolour - 1 - q;
         colour_phase(l, colour, s == 0 ? noise : NOISE_NONE, extp, tag);
       }
+    }
     if (C.m > 0) {
-      lr_dot(l, x[l].p);
+      lr_dot(l, X(l));
       int d = forward ? 1 : 0;
-      lr_update(x[l].p, Lv.g.cs, Lv.W[d], Lv.Sinv[d].p, noise, extlr, tag);
+      lr_update(X(l), Lv.g.cs, Lv.W[d], Lv.Sinv[d].p, noise, extlr, tag);
     }
   }
 
@@ -310,16 +405,16 @@ struct Engine : EngineBase {
     Level& Cc = *C.L[l + 1];
     long long nf = F.N * C.K, nc = Cc.N * C.K;
     if (F.g.dim == 3) {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, true>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, false>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
       launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
     } else {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, true>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, false>, x[l].p, f[l].p, r[l].p, F.g, F.st, C.K);
+      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
+      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
       launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
     }
     if (C.m > 0) {
-      lr_dot(l, x[l].p);
+      lr_dot(l, X(l));
       lr_update(f[l + 1].p, Cc.g.cs, Cc.Brows, C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
     }
   }
@@ -329,9 +424,9 @@ struct Engine : EngineBase {
     Level& Cc = *C.L[l + 1];
     long long nf = F.N * C.K;
     if (F.g.dim == 3)
-      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 3>, x[l].p, x[l + 1].p, F.g, Cc.g, (T)alpha, C.K);
+      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 3>, X(l), X(l + 1), F.g, Cc.g, (T)alpha, C.K);
     else
-      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 2>, x[l].p, x[l + 1].p, F.g, Cc.g, (T)alpha, C.K);
+      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 2>, X(l), X(l + 1), F.g, Cc.g, (T)alpha, C.K);
   }
 
   void coarse(int mode, const double* xi_dev) override {
@@ -341,17 +436,17 @@ struct Engine : EngineBase {
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
     if (mode == NOISE_NONE)
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                           (int)C.Nc, xi_dev, nk);
     else if (mode == NOISE_PHILOX)
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                             (int)C.Nc, xi_dev, nk);
     else
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, x[l].p, f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
+      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
                                                               C.Qinv.p, (int)C.Nc, xi_dev, nk);
   }
 
-  void zero_x(int l) { fill(x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0); }
+  void zero_x(int l) { fill(X(l), (long long)C.L[l]->g.cs * C.K, (T)0); }
 
   // MultigridMCSampler::sample (multigridmc_sampler.cc:109-136)
   void sample(int l) {
@@ -395,29 +490,31 @@ struct Engine : EngineBase {
   void cycle_sample() override {
     C.tag = 1;
     sample(0);
+    normalise();
   }
   void cycle_precondition() override {
     C.tag = 1;
     solve(0);
+    normalise();
   }
 
   void pack(int l, int which, const double* ref_dev, int per_chain) override {
     Level& Lv = *C.L[l];
-    T* dst = which == 0 ? x[l].p : which == 1 ? f[l].p : ext.p;
+    T* dst = which == 0 ? X(l) : which == 1 ? f[l].p : ext.p;
     launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_pack<T>, ref_dev, dst, Lv.g, C.K, per_chain);
   }
   void unpack(int l, int which, double* ref_dev, double scale) override {
     Level& Lv = *C.L[l];
-    const T* src = which == 0 ? x[l].p : which == 1 ? f[l].p : r[l].p;
+    const T* src = which == 0 ? X(l) : which == 1 ? f[l].p : r[l].p;
     launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_unpack<T>, src, ref_dev, Lv.g, C.K, scale);
   }
   void set_fine(int which, const double* ref_dev, int per_chain) override { pack(0, which, ref_dev, per_chain); }
   void get_fine_x(double* ref_dev, double scale) override { unpack(0, 0, ref_dev, scale); }
-  void* field(int l, int which) override { return which == 0 ? x[l].p : which == 1 ? f[l].p : r[l].p; }
+  void* field(int l, int which) override { return which == 0 ? X(l) : which == 1 ? f[l].p : r[l].p; }
 
   void probe(double* obs_dev) override {
     if (C.nprobe == 0 || !obs_dev) return;
-    launch(nblocks((long long)C.nprobe * C.K), 256, 0, C.stream, k_probe<T>, x[0].p, C.L[0]->g.cs, C.probe_pidx.p,
+    launch(nblocks((long long)C.nprobe * C.K), 256, 0, C.stream, k_probe<T>, X(0), C.L[0]->g.cs, C.probe_pidx.p,
                                                                           C.nprobe, C.K, obs_dev, C.ctr.p);
   }
 
@@ -443,6 +540,7 @@ struct Engine : EngineBase {
     if (noise == NOISE_EXTERNAL && C.m > 0 && !xilr_dev) throw Error(MGMC_ERR_ARG, "xi_lr needed with injected xi");
     if (noise == NOISE_PHILOX && xilr_dev) throw Error(MGMC_ERR_ARG, "xi_lr needs xi");
     dir_sweep(l, forward, noise, extp, xilr_dev);
+    normalise();
   }
 
   // ---- FP64 fine-level operations for the loop solver (T-independent data) ----
@@ -475,7 +573,7 @@ struct Engine : EngineBase {
   }
   void add_x_to_fp64(double* pad64) override {
     long long n = (long long)C.L[0]->g.cs * C.K;
-    launch(nblocks(n), 256, 0, C.stream, k_axpy_pad<T>, pad64, x[0].p, 1.0, n);
+    launch(nblocks(n), 256, 0, C.stream, k_axpy_pad<T>, pad64, X(0), 1.0, n);
   }
 };
 
@@ -934,6 +1032,24 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out) {
   launch(nblocks(n), 256, 0, C.stream, k_normals, d, C.L[0]->g, C.K, nk);
   HIP_CHECK(hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
   sync(h);
+  API_END
+}
+
+int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
+  API_BEGIN
+  if (!h || !key) throw Error(MGMC_ERR_ARG, "null argument");
+  std::string k(key);
+  if (k == "fused_sweep") h->C.tune_fused = value != 0;
+  else if (k == "rb_threads") {
+    if (value < 64 || value > 1024) throw Error(MGMC_ERR_ARG, "rb_threads must lie in [64, 1024]");
+    h->C.tune_rb_threads = value;
+  } else if (k == "rb_blocks") {
+    if (value < 1) throw Error(MGMC_ERR_ARG, "rb_blocks must be >= 1");
+    h->C.tune_rb_blocks = value;
+  } else
+    throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
+  h->E->retune();
+  h->graph_valid = 0;
   API_END
 }
 

@@ -96,6 +96,9 @@ class _Handle:
     def __del__(self):
         self.close()
 
+    def set_tuning(self, key, value):
+        check(_lib.lib().mgmc_set_tuning(self.h, key.encode(), int(value)))
+
     def ndof(self, level):
         n = C.c_int64()
         check(_lib.lib().mgmc_ndof(self.h, level, C.byref(n)))

@@ -175,3 +175,31 @@ def test_csr_operator_input_matches_fd():
             assert np.allclose(st, s_fd.level_stencil(level)[0], rtol=1e-14, atol=0)
     finally:
         _lib.lib().mgmc_destroy(h)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+@pytest.mark.parametrize("threads,blocks", [(512, 1024), (128, 1), (64, 100000)])
+def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
+    """The single-pass red-black kernel (tiles of full x-rows, z-segments with
+    redundant halo updates) equals two colour-phase launches, for Philox and
+    injected noise, in both directions, over several tile / segment shapes."""
+    op, _ = make_problem(3, (32, 16, 24), 3)
+    p = params(3)
+    K = 3
+    ref = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=5)
+    ref.set_tuning("fused_sweep", 0)
+    fus = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=5)
+    fus.set_tuning("rb_threads", threads)
+    fus.set_tuning("rb_blocks", blocks)
+    N = op.lattice.Nvertex
+    rng = np.random.default_rng(6)
+    b, x, xi = rng.standard_normal((3, K, N))
+    xl = rng.standard_normal((K, 3))
+    for fw in (True, False):
+        for kw in ({}, {"xi": xi, "xi_lr": xl}):
+            for st in (False, True):
+                if not st and kw:
+                    continue
+                a = ref.sweep(0, fw, st, b, x, **kw)
+                c = fus.sweep(0, fw, st, b, x, **kw)
+                assert np.abs(a - c).max() <= tol * np.abs(a).max(), (fw, st, bool(kw))
