@@ -157,7 +157,8 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out);
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);
 
 /* Launch-shape tuning (no reference counterpart): key "fused_sweep" (0/1:
- * sweep 3D star levels with the single-pass red-black kernel), "rb_threads"
+ * sweep 3D star levels with the single-pass red-black kernel), "plane_sweep"
+ * (0/1: sweep 3D box levels in two plane-parity launches), "rb_threads"
  * (threads per workgroup of that kernel, 64..1024), "rb_blocks" (workgroups
  * it aims for; more z-segments when fewer tiles x chains exist). Results do
  * not depend on these settings beyond rounding. */

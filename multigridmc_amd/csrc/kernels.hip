@@ -114,6 +114,77 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
   }
 }
 
+// Box-stencil update of the four parity-IPAR vertices of a group whose own
+// plane is in LDS (`row` = the group in LDS, xc its values with margins);
+// zm / zp point at the same group in the planes below / above (LDS or
+// global memory). Outer margins of the first / last group are never used.
+template <int IPAR, typename T, int NOISE>
+__device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* zm, const T* zp, const DevGeom& g,
+                                                  const Stencil& st, const T* fb, const float* xi, const T* extg,
+                                                  int i0, int gx, int ngx) {
+  T acc[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) acc[l] = (T)0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      T w[10];
+      if (dz == 0 && dy == 0) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) w[q] = xc[q];
+      } else {
+        const T* p = (dz < 0 ? zm : dz > 0 ? zp : row) + dy * g.px;
+        load8<T>(p, w + 1);
+        w[0] = gx > 0 ? p[-1] : (T)0;
+        w[9] = gx < ngx - 1 ? p[8] : (T)0;
+      }
+      const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
+      const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
+      const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int q = 2 * l + IPAR;
+        acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
+      }
+    }
+  }
+  const T om_d = (T)st.inv_diag_omega;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int q = 2 * l + IPAR;
+    const int i = i0 + q;
+    if (i < 1 || i > g.n0 - 1) continue;
+    T b = fb[q];
+    if (NOISE == NOISE_PHILOX) b += (T)st.noise_scale * (T)xi[l];
+    if (NOISE == NOISE_EXTERNAL) b += (T)st.noise_scale * extg[q];
+    xc[q + 1] += om_d * (b - acc[l]);
+  }
+}
+
+// Adds alpha * (P xc) to the fine 8-vertex group (8 gx .. 8 gx + 7, row j,
+// plane k) held in v: the linear interpolation of the coarse field xc
+// (chain base pointer cb, geometry gc) as in k_prolongate. Fine boundary
+// vertices receive only coarse boundary values, which are zero.
+template <typename T, int DIM>
+__device__ __forceinline__ void add_prolongation8(T* v, const T* cb, const DevGeom& gc, int gx, int j, int k,
+                                                  T alpha) {
+  const T half = (T)0.5;
+  const int j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
+  const int nz = (DIM == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1;
+  T s[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};
+  for (int a = 0; a < nz; ++a)
+    for (int b2 = 0; b2 < ny; ++b2) {
+      const int K = a ? k1 : k0, J = b2 ? j1 : j0;
+      const T* p = cb + ((long long)K * gc.py + J) * gc.px + 4 * gx;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) s[e] += (4 * gx + e < gc.px) ? p[e] : (T)0;
+    }
+  const T w = (nz == 2 ? half : (T)1) * (ny == 2 ? half : (T)1) * alpha;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] += w * ((q & 1) ? half * (s[q >> 1] + s[(q >> 1) + 1]) : s[q >> 1]);
+}
+
 // ---------------------------------------------------------------------------
 // One colour phase of an SOR sweep (SORSmoother::apply_sparse,
 // src/smoother/sor_smoother.cc:56-77, restricted to one colour), with the
@@ -198,43 +269,44 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
 }
 
 // ---------------------------------------------------------------------------
-// Low-rank (measurement) terms.
-// z[c][k] = sum_e B_ek x_c[idx_e]   (B^T x, FP64 accumulation)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
-                                                const long long* __restrict__ bidx, const double* __restrict__ bval,
-                                                int m, double* __restrict__ z) {
-  int kcol = blockIdx.x, c = blockIdx.y;
-  const T* xc = x + c * cs;
-  double s = 0.0;
-  for (int e = bptr[kcol] + threadIdx.x; e < bptr[kcol + 1]; e += blockDim.x) s += bval[e] * (double)xc[bidx[e]];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  __shared__ double red[4];
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) z[(long long)c * m + kcol] = red[0] + red[1] + red[2] + red[3];
-}
-
-// x_c[row_r] += sum_e val_e u_c[col_e] with u_c = M (a sqrt(Sigma) xi'_c + zc z_c).
+// Low-rank (measurement) terms, in two kernels.
+//
+// k_lr_dot: block (k, c) forms z[c][k] = sum_e B_ek x_c[idx_e] (B^T x, FP64
+// accumulation); k_lr_solve: u_c = M (a sqrt(Sigma) xi'_c + zc z_c) (m x m,
+// FP64, one block per chain); k_lr_update: x_c[row_r] += sum_e val_e u_c[col_e].
+//
 // Smoother:   M = S^{-1}, a = 0, zc = -1  ->  x -= W S^{-1} B^T x = B_bar B^T x
 //             (src/smoother/sor_smoother.cc:46-51).
 // Sampler:    M = S^{-1}, a = 1, zc = -1  ->  the SMW step with the sampler's
 //             low-rank noise B Sigma^{-1/2} xi' folded in
 //             (src/sampler/sor_sampler.cc:50-56): x' = y + W S^{-1} (Sigma^{1/2} xi' - B^T y).
 // Residual:   M = Sigma^{-1}, a = 0, zc = -1 -> subtract B Sigma^{-1} B^T x.
+// ---------------------------------------------------------------------------
 constexpr int MMAX = 128;
-template <typename T, int NOISE>
-__global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
-                                                   const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                                   const double* __restrict__ val, int nrows, const double* __restrict__ M,
-                                                   const double* __restrict__ sqrt_sigma, const double* __restrict__ z,
-                                                   double zc, int m, const double* __restrict__ ext_lr, NoiseKey nk) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
+                                                const long long* __restrict__ bidx, const double* __restrict__ bval,
+                                                int m, double* __restrict__ z) {
+  const int kcol = blockIdx.x, c = blockIdx.y;
+  const T* xc = x + c * cs;
+  double s = 0.0;
+  for (int e = bptr[kcol] + threadIdx.x; e < bptr[kcol + 1]; e += blockDim.x) s += bval[e] * (double)xc[bidx[e]];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) z[(long long)c * m + kcol] = red[0] + red[1] + red[2] + red[3];
+}
+
+// one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c)
+template <int NOISE>
+__global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z, const double* __restrict__ M,
+                                                   const double* __restrict__ sqrt_sigma, double zc,
+                                                   const double* __restrict__ ext_lr, double* __restrict__ u, int m,
+                                                   NoiseKey nk) {
   __shared__ double a[MMAX];
-  __shared__ double u[MMAX];
-  int c = blockIdx.y;
-  int tid = threadIdx.x;
+  const int c = blockIdx.x, tid = threadIdx.x;
   if (tid < m) {
     double v = zc * z[(long long)c * m + tid];
     if (NOISE == NOISE_PHILOX) {
@@ -247,121 +319,178 @@ __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long 
   }
   __syncthreads();
   if (tid < m) {
-    double s = 0.0;
-    for (int l = 0; l < m; ++l) s += M[tid * m + l] * a[l];
-    u[tid] = s;
+    double acc = 0.0;
+    for (int l = 0; l < m; ++l) acc += M[tid * m + l] * a[l];
+    u[(long long)c * m + tid] = acc;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
+                                                   const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                   const double* __restrict__ val, int nrows,
+                                                   const double* __restrict__ u, int m) {
+  __shared__ double us[MMAX];
+  const int c = blockIdx.y;
+  const int tid = threadIdx.x;
+  if (tid < m) us[tid] = u[(long long)c * m + tid];
   __syncthreads();
-  int r = blockIdx.x * blockDim.x + tid;
+  const int r = blockIdx.x * blockDim.x + tid;
   if (r >= nrows) return;
   double s = 0.0;
-  for (int e = ptr[r]; e < ptr[r + 1]; ++e) s += val[e] * u[col[e]];
+  for (int e = ptr[r]; e < ptr[r + 1]; ++e) s += val[e] * us[col[e]];
   T* p = x + c * cs + rows[r];
   *p = (T)((double)*p + s);
 }
 
 // ---------------------------------------------------------------------------
+// Transfers. Grids are (groups x rows, chains): thread t of chain
+// blockIdx.y owns group t % ng of interior row t / ng, row id
+// (j - 1) + (n1 - 1)(k - 1); an aligned 8-vertex group (vector loads and
+// stores, 32-bit index arithmetic only).
+// ---------------------------------------------------------------------------
+
 // r = f - A x on interior vertices (the sparse part of
 // src/linear_operator/linear_operator.hh:78-88, multigridmc_sampler.cc:125-126)
-// ---------------------------------------------------------------------------
 template <typename T, int DIM, bool STAR>
 __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const T* __restrict__ f, T* __restrict__ r,
-                                                  DevGeom g, Stencil st, int K) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long per = (long long)(g.n0 - 1) * (g.n1 - 1) * (DIM == 3 ? g.n2 - 1 : 1);
-  if (t >= per * K) return;
-  int c = (int)(t / per);
-  long long e = t % per;
-  int i = (int)(e % (g.n0 - 1)) + 1;
-  e /= (g.n0 - 1);
-  int j = (int)(e % (g.n1 - 1)) + 1;
-  int k = DIM == 3 ? (int)(e / (g.n1 - 1)) + 1 : 0;
+                                                  DevGeom g, Stencil st) {
+  const int ngx = g.px >> 3;
+  const int nrows = (g.n1 - 1) * (DIM == 3 ? g.n2 - 1 : 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ngx * nrows) return;
+  const int gx = t % ngx, row = t / ngx, c = blockIdx.y;
+  const int j = row % (g.n1 - 1) + 1;
+  const int k = DIM == 3 ? row / (g.n1 - 1) + 1 : 0;
   const long long pl = (long long)g.px * g.py;
-  const long long p = c * g.cs + ((long long)k * g.py + j) * g.px + i;
-  T acc;
+  const long long base = (long long)c * g.cs + ((long long)k * g.py + j) * g.px + 8 * gx;
+  const T* xr = x + base;
+  T xc[10];
+  load8<T>(xr, xc + 1);
+  xc[0] = xr[-1];
+  xc[9] = xr[8];
+  T acc[8];
   if (STAR) {
-    // stencil index (dz+1)*9 + (dy+1)*3 + (dx+1): x-+ 12/14, y-+ 10/16, z-+ 4/22
-    acc = (T)st.c[10] * x[p - g.px] + (T)st.c[12] * x[p - 1] + (T)st.diag * x[p] + (T)st.c[14] * x[p + 1] +
-          (T)st.c[16] * x[p + g.px];
-    if (DIM == 3) acc = (T)st.c[4] * x[p - pl] + acc + (T)st.c[22] * x[p + pl];
+    T ym[8], yp[8];
+    load8<T>(xr - g.px, ym);
+    load8<T>(xr + g.px, yp);
+    const T cx = (T)st.c[12], cy = (T)st.c[10], D = (T)st.diag;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = cy * ym[q] + cx * xc[q] + D * xc[q + 1] + cx * xc[q + 2] + cy * yp[q];
+    if (DIM == 3) {
+      T zm[8], zp[8];
+      load8<T>(xr - pl, zm);
+      load8<T>(xr + pl, zp);
+      const T cz = (T)st.c[4];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] = cz * zm[q] + acc[q] + cz * zp[q];
+    }
   } else {
-    acc = (T)0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = (T)0;
 #pragma unroll
     for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
 #pragma unroll
-      for (int dy = -1; dy <= 1; ++dy)
+      for (int dy = -1; dy <= 1; ++dy) {
+        T w[10];
+        const T* p = xr + dz * pl + dy * g.px;
+        load8<T>(p, w + 1);
+        w[0] = p[-1];
+        w[9] = p[8];
+        const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
+        const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
+        const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx)
-          acc += (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] * x[p + dz * pl + dy * g.px + dx];
+        for (int q = 0; q < 8; ++q) acc[q] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
+      }
   }
-  r[p] = (f ? f[p] : (T)0) - acc;
+  T fb[8], out[8];
+  if (f) load8<T>(f + base, fb);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = 8 * gx + q;
+    out[q] = (i >= 1 && i <= g.n0 - 1) ? (f ? fb[q] : (T)0) - acc[q] : (T)0;
+  }
+  store8<T>(r + base, out);
 }
 
 // fc[I] = sum_sigma p(sigma) r[2I + sigma], p = prod {1/2, 1, 1/2}
-// (IntergridOperator::restrict, src/intergrid/intergrid_operator.hh:74-88)
+// (IntergridOperator::restrict, src/intergrid/intergrid_operator.hh:74-88).
+// A thread owns four coarse vertices I = 4 gq .. 4 gq + 3 of a coarse row,
+// i.e. fine x positions 8 gq - 1 .. 8 gq + 7.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc,
-                                                  int K) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long per = (long long)(gc.n0 - 1) * (gc.n1 - 1) * (DIM == 3 ? gc.n2 - 1 : 1);
-  if (t >= per * K) return;
-  int c = (int)(t / per);
-  long long e = t % per;
-  int I = (int)(e % (gc.n0 - 1)) + 1;
-  e /= (gc.n0 - 1);
-  int J = (int)(e % (gc.n1 - 1)) + 1;
-  int Kz = DIM == 3 ? (int)(e / (gc.n1 - 1)) + 1 : 0;
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc) {
+  const int nq = (gc.n0 - 1 + 4) / 4;  // groups of four coarse vertices covering I = 0 .. n0c - 1
+  const int nrows = (gc.n1 - 1) * (DIM == 3 ? gc.n2 - 1 : 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq * nrows) return;
+  const int gq = t % nq, row = t / nq, c = blockIdx.y;
+  const int J = row % (gc.n1 - 1) + 1;
+  const int Kz = DIM == 3 ? row / (gc.n1 - 1) + 1 : 0;
   const long long pl = (long long)gf.px * gf.py;
-  const long long p = c * gf.cs + ((long long)(2 * Kz) * gf.py + 2 * J) * gf.px + 2 * I;
+  const long long fb0 = (long long)c * gf.cs + ((long long)(2 * Kz) * gf.py + 2 * J) * gf.px + 8 * gq;
   const T w[3] = {(T)0.5, (T)1.0, (T)0.5};
-  T acc = (T)0;
+  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
   for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
-      const T* q = r + p + dz * pl + dy * gf.px;
-      T wz = DIM == 3 ? w[dz + 1] : (T)1;
-      acc += wz * w[dy + 1] * ((T)0.5 * q[-1] + q[0] + (T)0.5 * q[1]);
+      const T* q = r + fb0 + dz * pl + dy * gf.px;
+      T v[10];
+      load8<T>(q, v + 1);
+      v[0] = q[-1];
+      v[9] = q[8];
+      const T wzy = (DIM == 3 ? w[dz + 1] : (T)1) * w[dy + 1];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] += wzy * ((T)0.5 * v[2 * a] + v[2 * a + 1] + (T)0.5 * v[2 * a + 2]);
     }
-  fc[c * gc.cs + ((long long)Kz * gc.py + J) * gc.px + I] = acc;
+  T* out = fc + (long long)c * gc.cs + ((long long)Kz * gc.py + J) * gc.px + 4 * gq;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int I = 4 * gq + a;
+    if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a];
+  }
 }
 
 // x += alpha P xc (IntergridOperator::prolongate_add in gather form,
-// src/intergrid/intergrid_operator.hh:106-120)
+// src/intergrid/intergrid_operator.hh:106-120). A thread owns an aligned
+// 8-vertex fine group, whose coarse neighbours are I = 4 gx .. 4 gx + 4.
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* __restrict__ xc, DevGeom gf, DevGeom gc,
-                                                    T alpha, int K) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long per = (long long)(gf.n0 - 1) * (gf.n1 - 1) * (DIM == 3 ? gf.n2 - 1 : 1);
-  if (t >= per * K) return;
-  int c = (int)(t / per);
-  long long e = t % per;
-  int i = (int)(e % (gf.n0 - 1)) + 1;
-  e /= (gf.n0 - 1);
-  int j = (int)(e % (gf.n1 - 1)) + 1;
-  int k = DIM == 3 ? (int)(e / (gf.n1 - 1)) + 1 : 0;
-  int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
-  T wi = (i & 1) ? (T)0.5 : (T)1, wj = (j & 1) ? (T)0.5 : (T)1, wk = (DIM == 3 && (k & 1)) ? (T)0.5 : (T)1;
-  const T* b = xc + c * gc.cs;
-  auto at = [&](int a, int bb, int cc) { return b[((long long)cc * gc.py + bb) * gc.px + a]; };
-  T s;
-  if (DIM == 3) {
-    T s0 = at(i0, j0, k0) + ((i & 1) ? at(i1, j0, k0) : (T)0);
-    T s1 = (j & 1) ? at(i0, j1, k0) + ((i & 1) ? at(i1, j1, k0) : (T)0) : (T)0;
-    T sa = s0 + s1;
-    T sb = (T)0;
-    if (k & 1) {
-      T s2 = at(i0, j0, k1) + ((i & 1) ? at(i1, j0, k1) : (T)0);
-      T s3 = (j & 1) ? at(i0, j1, k1) + ((i & 1) ? at(i1, j1, k1) : (T)0) : (T)0;
-      sb = s2 + s3;
+                                                    T alpha) {
+  const int ngx = gf.px >> 3;
+  const int nrows = (gf.n1 - 1) * (DIM == 3 ? gf.n2 - 1 : 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ngx * nrows) return;
+  const int gx = t % ngx, row = t / ngx, c = blockIdx.y;
+  const int j = row % (gf.n1 - 1) + 1;
+  const int k = DIM == 3 ? row / (gf.n1 - 1) + 1 : 0;
+  // coarse rows/planes touching fine row j / plane k, with weights
+  const int j0 = j >> 1, j1 = (j + 1) >> 1;
+  const int k0 = k >> 1, k1 = (k + 1) >> 1;
+  const T half = (T)0.5;
+  const T* cb = xc + (long long)c * gc.cs;
+  T s[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};  // y/z-interpolated coarse values at I = 4 gx .. 4 gx + 4
+  const int nz = (DIM == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1;
+  const T wz = nz == 2 ? half : (T)1, wy = ny == 2 ? half : (T)1;
+  for (int a = 0; a < nz; ++a)
+    for (int b2 = 0; b2 < ny; ++b2) {
+      const int K = a ? k1 : k0, J = b2 ? j1 : j0;
+      const T* p = cb + ((long long)K * gc.py + J) * gc.px + 4 * gx;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) s[e] += (4 * gx + e < gc.px) ? p[e] : (T)0;
     }
-    s = wi * wj * wk * (sa + sb);
-  } else {
-    T s0 = at(i0, j0, 0) + ((i & 1) ? at(i1, j0, 0) : (T)0);
-    T s1 = (j & 1) ? at(i0, j1, 0) + ((i & 1) ? at(i1, j1, 0) : (T)0) : (T)0;
-    s = wi * wj * (s0 + s1);
+  const T w = wz * wy * alpha;
+  T* xr = x + (long long)c * gf.cs + ((long long)k * gf.py + j) * gf.px + 8 * gx;
+  T v[8];
+  load8<T>(xr, v);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = 8 * gx + q;
+    const T p = (q & 1) ? half * (s[q >> 1] + s[(q >> 1) + 1]) : s[q >> 1];
+    if (i >= 1 && i <= gf.n0 - 1) v[q] += w * p;
   }
-  x[c * gf.cs + ((long long)k * gf.py + j) * gf.px + i] += alpha * s;
+  store8<T>(xr, v);
 }
 
 template <typename T>
@@ -410,17 +539,22 @@ __global__ void __launch_bounds__(256) k_unpack(const T* __restrict__ pad, doubl
 
 // Coarsest level: x = L^{-T} xi + Q^{-1} f (= L^{-T}(xi + L^{-1} f),
 // src/sampler/cholesky_sampler.hh:52-66) or x = Q^{-1} f
-// (src/solver/cholesky_solver.cc:30-40); FP64 inside. One block per chain.
+// (src/solver/cholesky_solver.cc:30-40); FP64 inside. Grid (row blocks of 64,
+// chains); the four waves of a block split the inner dimension and reduce
+// through LDS. The matrices are stored column-major (element (r, j) at
+// j * Nc + r) so that a wave reads 64 consecutive rows of one column.
 template <typename T, int NOISE>
 __global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs,
-                                                const long long* __restrict__ pidx, const double* __restrict__ LinvT,
-                                                const double* __restrict__ Qinv, int Nc, const double* __restrict__ ext,
+                                                const long long* __restrict__ pidx, const double* __restrict__ LT,
+                                                const double* __restrict__ Q, int Nc, const double* __restrict__ ext,
                                                 NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(double, sh);
   double* fs = sh;
   double* xs = sh + Nc;
-  int c = blockIdx.x;
-  for (int j = threadIdx.x; j < Nc; j += blockDim.x) {
+  __shared__ double red[4][64];
+  const int c = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < Nc; j += blockDim.x) {
     fs[j] = (double)f[c * cs + pidx[j]];
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
@@ -431,16 +565,18 @@ __global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __re
     }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < Nc; r += blockDim.x) {
-    double s = 0.0;
-    const double* q = Qinv + (long long)r * Nc;
-    for (int j = 0; j < Nc; ++j) s += q[j] * fs[j];
-    if (NOISE != NOISE_NONE) {
-      const double* l = LinvT + (long long)r * Nc;
-      for (int j = r; j < Nc; ++j) s += l[j] * xs[j];
-    }
-    x[c * cs + pidx[r]] = (T)s;
+  const int lane = tid & 63, part = tid >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int j0 = (Nc * part) / 4, j1 = (Nc * (part + 1)) / 4;
+  double s = 0.0;
+  if (r < Nc) {
+    for (int j = j0; j < j1; ++j) s += Q[(long long)j * Nc + r] * fs[j];
+    if (NOISE != NOISE_NONE)
+      for (int j = j0 > r ? j0 : r; j < j1; ++j) s += LT[(long long)j * Nc + r] * xs[j];
   }
+  red[part][lane] = s;
+  __syncthreads();
+  if (part == 0 && r < Nc) x[c * cs + pidx[r]] = (T)(red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +601,8 @@ template <typename T, int NOISE>
 __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
-                                                    NoiseKey nk) {
+                                                    NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
+                                                    T alpha) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
   const size_t asz = (size_t)(TY + 4) * g.px, bsz = (size_t)(TY + 2) * g.px;
   T* A0 = reinterpret_cast<T*>(lds_raw);
@@ -494,8 +631,16 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
   T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = (T)0;
-  if (has_mem && ks - 2 >= 0) load8<T>(x + cbase + (ks - 2) * pl, xm);
-  if (has_mem) load8<T>(x + cbase + (ks - 1) * pl, xc);
+  // optional prolongate-add of the coarse correction while the planes are read
+  const T* cb = xcoarse ? xcoarse + (long long)chain * gc.cs : nullptr;
+  if (has_mem && ks - 2 >= 0) {
+    load8<T>(x + cbase + (ks - 2) * pl, xm);
+    if (cb) add_prolongation8<T, 3>(xm, cb, gc, gx, j, ks - 2, alpha);
+  }
+  if (has_mem) {
+    load8<T>(x + cbase + (ks - 1) * pl, xc);
+    if (cb) add_prolongation8<T, 3>(xc, cb, gc, gx, j, ks - 1, alpha);
+  }
 
   const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
   const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
@@ -507,8 +652,10 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
     const bool doA = ring && k >= 1 && k <= g.n2 - 1;
     const bool doB = owned && kb >= ks && kb <= ke - 1;
     // loads first: next plane of x, f on plane k
-    if (has_mem && k + 1 <= g.n2) load8<T>(x + cbase + (long long)(k + 1) * pl, xp);
-    else {
+    if (has_mem && k + 1 <= g.n2) {
+      load8<T>(x + cbase + (long long)(k + 1) * pl, xp);
+      if (cb) add_prolongation8<T, 3>(xp, cb, gc, gx, j, k + 1, alpha);
+    } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q) xp[q] = (T)0;
     }
@@ -568,6 +715,196 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
       xm[q] = xc[q];
       xc[q] = xp[q];
       fm[q] = fc[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Half of an 8-colour SOR/Gibbs sweep of a 3D box-stencil (27-point) level:
+// every interior plane of z-parity `kpar` gets its four colours (ipar, jpar)
+// in sweep order. Planes of equal parity do not couple (the stencil reaches
+// k +- 1 only), so one workgroup per (plane, chain) reproduces the colour-
+// phase launches of k_sweep for colours 4 kpar .. 4 kpar + 3 (forward) or the
+// reverse (backward). Plane k lives in LDS; planes k +- 1 are read-only here.
+// One thread per aligned 8-vertex group of a row.
+// ---------------------------------------------------------------------------
+template <typename T, int NOISE, bool STAGE3>
+__global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
+                                                           const T* __restrict__ ext, DevGeom g, Stencil st,
+                                                           int kpar, int forward, int nplanes, NoiseKey nk) {
+  MGMC_DYNAMIC_SHARED(char, lds_raw);
+  // STAGE3: planes k-1, k, k+1 in LDS (py x px each); else plane k only
+  const size_t psz = (size_t)g.py * g.px;
+  T* P = reinterpret_cast<T*>(lds_raw) + (STAGE3 ? psz : 0);
+  const int ngx = g.px >> 3;
+  const int tid = threadIdx.x;
+  const int r = tid / ngx, gx = tid % ngx;
+  const int j = r + 1;
+  const bool active = j <= g.n1 - 1;
+  const int pidx = blockIdx.x % nplanes;
+  const int chain = blockIdx.x / nplanes;
+  const int k = kpar ? 2 * pidx + 1 : 2 * pidx + 2;
+  const long long base = (long long)chain * g.cs + ((long long)k * g.py + j) * g.px + 8 * gx;
+  // stage plane k (and k +- 1), boundary rows 0 and n1 included, in LDS
+  for (int dz = STAGE3 ? -1 : 0; dz <= (STAGE3 ? 1 : 0); ++dz)
+    for (int e = tid; e < g.py * ngx; e += blockDim.x) {
+      const int rr = e / ngx, gg = e % ngx;
+      T v[8];
+      load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, v);
+      store8<T>(P + dz * (long long)psz + (size_t)rr * g.px + 8 * gg, v);
+    }
+  T fb[8];
+  if (active) load8<T>(f + base, fb);
+  const uint32_t sample = *nk.sample;
+  const int jpar = j & 1;
+  const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
+  for (int ph = 0; ph < 4; ++ph) {
+    const int q = forward ? ph : 3 - ph;  // in-plane colour: ipar = q & 1, jpar = q >> 1
+    __syncthreads();
+    if (!active || jpar != (q >> 1)) continue;
+    const int ipar = q & 1;
+    T* row = P + (size_t)j * g.px + 8 * gx;
+    T xc[10];
+    load8<T>(row, xc + 1);
+    xc[0] = gx > 0 ? row[-1] : (T)0;
+    xc[9] = gx < ngx - 1 ? row[8] : (T)0;
+    float xi[4] = {0.f, 0.f, 0.f, 0.f};
+    if (NOISE == NOISE_PHILOX) {
+      uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + gx) << 1) | (uint32_t)ipar;
+      normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xi);
+    }
+    const T* zm = STAGE3 ? row - psz : x + base - (long long)g.px * g.py;
+    const T* zp = STAGE3 ? row + psz : x + base + (long long)g.px * g.py;
+    if (ipar == 0)
+      box_plane_update4<0, T, NOISE>(xc, row, zm, zp, g, st, fb, xi, eg, 8 * gx, gx, ngx);
+    else
+      box_plane_update4<1, T, NOISE>(xc, row, zm, zp, g, st, fb, xi, eg, 8 * gx, gx, ngx);
+    store8<T>(row, xc + 1);
+  }
+  __syncthreads();
+  if (active) {
+    T v[8];
+    load8<T>(P + (size_t)j * g.px + 8 * gx, v);
+    store8<T>(x + base, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused residual and restriction on a 3D star-stencil level:
+//   fc = R (f - A x)   (multigridmc_sampler.cc:125-128 without the low-rank
+//   term, which is added by the low-rank kernels).
+// A workgroup owns TYc coarse rows (fine rows 2 J0 - 1 .. 2 J0 + 2 TYc - 1
+// plus one more x row on each side) and marches over the fine planes of its
+// coarse z-segment [Ks, Ke): r(p) is formed from plane p in LDS and planes
+// p +- 1 in registers, restricted in x (left neighbour from LDS), in y (LDS)
+// and accumulated in z; coarse plane K is stored when fine plane 2K + 1 is
+// done. One thread per fine 8-vertex group and fine row.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
+                                                                  T* __restrict__ fc, DevGeom g, DevGeom gc,
+                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg) {
+  MGMC_DYNAMIC_SHARED(char, lds_raw);
+  const int nrx = 2 * TYc + 3, nrr = 2 * TYc + 1;
+  T* LX = reinterpret_cast<T*>(lds_raw);    // nrx rows x px: x plane p
+  T* LR = LX + (size_t)nrx * g.px;          // nrr rows x px: r plane p
+  T* LT = LR + (size_t)nrr * g.px;          // nrr rows x (px / 2): x-restricted r
+  const int ngx = g.px >> 3;
+  const int tid = threadIdx.x;
+  const int rt = tid / ngx, gx = tid % ngx;
+  int b = blockIdx.x;
+  const int tile = b % ntile;
+  b /= ntile;
+  const int seg = b % nseg;
+  const int chain = b / nseg;
+  const int J0 = 1 + tile * TYc;
+  const int j = 2 * J0 - 2 + rt;
+  const bool active = rt < nrx;
+  const bool has_mem = active && j >= 0 && j <= g.n1;
+  const bool rrow = active && rt >= 1 && rt <= nrr;                 // rows whose residual enters the tile
+  const bool rvalid = rrow && j >= 1 && j <= g.n1 - 1;
+  const int J = j >> 1;
+  const bool crow = rrow && !(j & 1) && J >= J0 && J < J0 + TYc && J <= gc.n1 - 1;
+  const int Ks = 1 + seg * SZc;
+  const int Ke = Ks + SZc < gc.n2 ? Ks + SZc : gc.n2;  // coarse planes [Ks, Ke)
+  const long long pl = (long long)g.px * g.py;
+  const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
+  const T half = (T)0.5;
+
+  T xm[8], xc[8], xp[8], fb[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = fb[q] = (T)0;
+  const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;
+  if (has_mem) {
+    load8<T>(x + cbase + (long long)(p0 - 1) * pl, xm);
+    load8<T>(x + cbase + (long long)p0 * pl, xc);
+  }
+  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+  T* LXrow = LX + (size_t)rt * g.px + 8 * gx;
+  T* LRrow = LR + (size_t)(rt - 1) * g.px + 8 * gx;
+  T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
+
+  for (int p = p0; p <= p1; ++p) {
+    if (has_mem) load8<T>(x + cbase + (long long)(p + 1) * pl, xp);
+    if (rvalid) load8<T>(f + cbase + (long long)p * pl, fb);
+    if (active) store8<T>(LXrow, xc);
+    __syncthreads();
+    T r[8];
+    if (rrow) {
+      T ym[8], yp[8];
+      load8<T>(LXrow - g.px, ym);
+      load8<T>(LXrow + g.px, yp);
+      const T left = gx > 0 ? LXrow[-1] : (T)0;
+      const T right = gx < ngx - 1 ? LXrow[8] : (T)0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const T xl = q > 0 ? xc[q - 1] : left;
+        const T xr = q < 7 ? xc[q + 1] : right;
+        T a = cy * ym[q] + cx * xl + D * xc[q] + cx * xr + cy * yp[q];
+        a = cz * xm[q] + a + cz * xp[q];
+        const int i = 8 * gx + q;
+        r[q] = (rvalid && i >= 1 && i <= g.n0 - 1) ? fb[q] - a : (T)0;
+      }
+      store8<T>(LRrow, r);
+    }
+    __syncthreads();
+    if (rrow) {
+      const T rl = gx > 0 ? LRrow[-1] : (T)0;
+      T t[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) t[a] = half * (a > 0 ? r[2 * a - 1] : rl) + r[2 * a] + half * r[2 * a + 1];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) LTrow[a] = t[a];
+    }
+    __syncthreads();
+    if (crow) {
+      const int hp = g.px >> 1;
+      T u[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) u[a] = half * LTrow[a - hp] + LTrow[a] + half * LTrow[a + hp];
+      if (p & 1) {
+        // fine plane 2K+1 completes coarse plane K = (p-1)/2 and starts (p+1)/2
+        const int K = (p - 1) >> 1;
+        if (K >= Ks) {
+          T* out = fc + (long long)chain * gc.cs + ((long long)K * gc.py + J) * gc.px + 4 * gx;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const int I = 4 * gx + a;
+            if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a] + half * u[a];
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] = half * u[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] += u[a];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      xm[q] = xc[q];
+      xc[q] = xp[q];
     }
   }
 }

@@ -111,7 +111,7 @@ struct Common {
   uint32_t k0 = 0, k1 = 0;
   HostHierarchy H;
   std::vector<std::unique_ptr<Level>> L;
-  DevBuf<double> sqrt_sigma, sigma_inv_mat, z;
+  DevBuf<double> sqrt_sigma, sigma_inv_mat, z, u;
   DevBuf<long long> cpidx;
   DevBuf<double> LinvT, Qinv;
   int64_t Nc = 0;
@@ -124,6 +124,12 @@ struct Common {
   int tune_fused = 1;
   int tune_rb_threads = 512;
   int tune_rb_blocks = 1024;
+  int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
+  int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
+  int tune_rr_threads = 512;
+  int tune_planes = 1;        // plane-parity sweeps on 3D box levels
+  int tune_planes_min = 256;  // ... when planes x chains reaches this many workgroups
+  int tune_planes_stage3 = 1; // ... with planes k-1, k, k+1 staged in LDS when they fit
   // optional HIP-event timing of the level-0 colour-phase launches
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -190,6 +196,8 @@ struct Engine : EngineBase {
     r.resize(C.nlevel);
     x2.resize(C.nlevel);
     cur.assign(C.nlevel, 0);
+    pending.assign(C.nlevel, 0);
+    pending_alpha.assign(C.nlevel, 0.0);
     for (int l = 0; l < C.nlevel; ++l) {
       size_t n = (size_t)C.L[l]->g.cs * C.K;
       x[l].alloc(n, C.stream);
@@ -233,6 +241,10 @@ struct Engine : EngineBase {
     s.lds = (size_t)2 * (2 * s.TY + 6) * g.px * sizeof(T);
     return s;
   }
+  // prolongate-add pending for the next fused sweep of a level (folded into its loads)
+  std::vector<double> pending_alpha;
+  std::vector<int> pending;
+
   void fused_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
     Level& Lv = *C.L[l];
     RbShape s = rb_shape(l);
@@ -242,15 +254,24 @@ struct Engine : EngineBase {
     dim3 grid((unsigned)(s.ntile * s.nseg * C.K));
     int first = forward ? 0 : 1;
     const bool timed = prof_begin(l);
+    const T* xcoarse = nullptr;
+    DevGeom gc = Lv.g;
+    T alpha = (T)0;
+    if (pending[l]) {
+      xcoarse = X(l + 1);
+      gc = C.L[l + 1]->g;
+      alpha = (T)pending_alpha[l];
+      pending[l] = 0;
+    }
     if (noise == NOISE_NONE)
       launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, Lv.g, Lv.st, first,
-             s.TY, s.SZ, s.ntile, s.nseg, nk);
+             s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     else if (noise == NOISE_PHILOX)
       launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_PHILOX>, in, out, f[l].p, extp, Lv.g, Lv.st,
-             first, s.TY, s.SZ, s.ntile, s.nseg, nk);
+             first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     else
       launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_EXTERNAL>, in, out, f[l].p, extp, Lv.g, Lv.st,
-             first, s.TY, s.SZ, s.ntile, s.nseg, nk);
+             first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     if (timed) prof_end();
     cur[l] ^= 1;
   }
@@ -259,6 +280,44 @@ struct Engine : EngineBase {
       if (fused_ok(l) && !x2[l].p) x2[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
     HIP_CHECK(hipStreamSynchronize(C.stream));
   }
+  // ---- plane-parity sweep of 3D box levels (k_sweep_box_planes) ----
+  bool planes_ok(int l) const {
+    const Level& Lv = *C.L[l];
+    if (!C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
+    const DevGeom& g = Lv.g;
+    // one workgroup per (plane, chain): worth it only with enough workgroups
+    return (g.n1 - 1) * (g.px >> 3) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= 65536 &&
+           (long long)((g.n2 - 1) / 2) * C.K >= C.tune_planes_min;
+  }
+  void plane_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
+    Level& Lv = *C.L[l];
+    const DevGeom& g = Lv.g;
+    const int threads = (((g.n1 - 1) * (g.px >> 3) + 63) / 64) * 64;
+    const size_t plane = (size_t)g.py * g.px * sizeof(T);
+    const bool stage3 = C.tune_planes_stage3 && 3 * plane <= 65536;
+    const size_t lds = stage3 ? 3 * plane : plane;
+    NoiseKey nk = key(tag);
+    for (int h = 0; h < 2; ++h) {
+      const int kpar = forward ? h : 1 - h;
+      const int nplanes = kpar ? g.n2 / 2 : (g.n2 - 1) / 2;
+      if (nplanes <= 0) continue;
+      dim3 grid((unsigned)(nplanes * C.K));
+#define MGMC_PLANES(NOISE, S3)                                                                                  \
+  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, g, Lv.st, kpar, \
+         (int)forward, nplanes, nk)
+      if (stage3) {
+        if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, true);
+        else if (noise == NOISE_PHILOX) MGMC_PLANES(NOISE_PHILOX, true);
+        else MGMC_PLANES(NOISE_EXTERNAL, true);
+      } else {
+        if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, false);
+        else if (noise == NOISE_PHILOX) MGMC_PLANES(NOISE_PHILOX, false);
+        else MGMC_PLANES(NOISE_EXTERNAL, false);
+      }
+#undef MGMC_PLANES
+    }
+  }
+
   // bring every level's state back into x (end of a cycle / before host access)
   void normalise() {
     for (int l = 0; l < C.nlevel; ++l)
@@ -270,6 +329,12 @@ struct Engine : EngineBase {
   }
 
   NoiseKey key(uint32_t tag) const { return NoiseKey{C.k0, C.k1, tag, C.ctr.p}; }
+
+  // grid of the row-based transfer kernels: `groups` threads per interior row
+  dim3 row_grid(const DevGeom& g, int groups) const {
+    long long rows = (long long)(g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1);
+    return dim3(nblocks(rows * groups), C.K);
+  }
 
   void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, k_fill<T>, p, n, v); }
 
@@ -329,25 +394,26 @@ struct Engine : EngineBase {
     if (timed) prof_end();
   }
 
-  void lr_dot(int l, const T* xp) {
+  // u = M (a sqrt(Sigma) xi' - B_l^T x) for every chain (k_lr_dot + k_lr_solve)
+  void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
     dim3 grid(C.m, C.K);
     launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.z.p);
-  }
-
-  void lr_update(T* xp, long long cs, DevRows& R, const double* M, int noise, const double* extlr, uint32_t tag) {
-    if (R.nrows == 0) return;
-    dim3 grid(nblocks(R.nrows), C.K);
     NoiseKey nk = key(tag);
     if (noise == NOISE_NONE)
-      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_NONE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, M,
-                                                             C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
     else if (noise == NOISE_PHILOX)
-      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_PHILOX>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, M,
-                                                               C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
     else
-      launch(grid, 256, 0, C.stream, k_lr_update<T, NOISE_EXTERNAL>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows,
-                                                                 M, C.sqrt_sigma.p, C.z.p, -1.0, C.m, extlr, nk);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
+             nk);
+  }
+
+  // x[rows(R)] += R u
+  void lr_apply(T* xp, long long cs, DevRows& R) {
+    if (R.nrows == 0) return;
+    dim3 grid(nblocks(R.nrows), C.K);
+    launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m);
   }
 
   // One directional SOR sweep (+ SMW), SORSmoother::apply / SORSampler::apply
@@ -361,15 +427,19 @@ struct Engine : EngineBase {
         fused_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
         continue;
       }
+      if (planes_ok(l)) {
+        plane_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
+        continue;
+      }
       for (int q = 0; q < Lv.ncolour; ++q) {
         int colour = forward ? q : Lv.ncolour - 1 - q;
         colour_phase(l, colour, s == 0 ? noise : NOISE_NONE, extp, tag);
       }
     }
     if (C.m > 0) {
-      lr_dot(l, X(l));
       int d = forward ? 1 : 0;
-      lr_update(X(l), Lv.g.cs, Lv.W[d], Lv.Sinv[d].p, noise, extlr, tag);
+      lr_dot_solve(l, X(l), Lv.Sinv[d].p, noise, extlr, tag);
+      lr_apply(X(l), Lv.g.cs, Lv.W[d]);
     }
   }
 
@@ -399,34 +469,91 @@ struct Engine : EngineBase {
     }
   }
 
+  // fused residual + restriction on 3D star levels (k_residual_restrict_star3)
+  bool rr_ok(int l) const {
+    const Level& Lv = *C.L[l];
+    if (!C.tune_fused_rr || !Lv.star || Lv.g.dim != 3) return false;
+    return rr_tyc(l) >= 1;
+  }
+  int rr_tyc(int l) const {
+    const DevGeom& g = C.L[l]->g;
+    const DevGeom& gc = C.L[l + 1]->g;
+    int ngx = g.px >> 3;
+    int by_threads = (C.tune_rr_threads / ngx - 3) / 2;
+    // LDS: (2TYc+3 + 2TYc+1) px + (2TYc+1) px/2 values <= 64 KB
+    int by_lds = (int)(((65536 / sizeof(T)) / g.px * 2 - 9) / 10);
+    int t = std::min(std::min(by_threads, by_lds), gc.n1 - 1);
+    if (t < 1) return 0;
+    const int ntile = (gc.n1 - 1 + t - 1) / t;  // balance the tiles
+    return (gc.n1 - 1 + ntile - 1) / ntile;
+  }
+  void residual_restrict_fused(int l) {
+    Level& F = *C.L[l];
+    Level& Cc = *C.L[l + 1];
+    const DevGeom& g = F.g;
+    const DevGeom& gc = Cc.g;
+    const int TYc = rr_tyc(l);
+    const int ntile = (gc.n1 - 1 + TYc - 1) / TYc;
+    const int ncpl = gc.n2 - 1;
+    int want = (C.tune_rb_blocks + ntile * C.K - 1) / (ntile * C.K);
+    int nseg = std::max(1, std::min(want, std::max(1, ncpl / 4)));
+    const int SZc = (ncpl + nseg - 1) / nseg;
+    nseg = (ncpl + SZc - 1) / SZc;
+    const int ngx = g.px >> 3;
+    const int threads = ((ngx * (2 * TYc + 3) + 63) / 64) * 64;
+    const size_t lds = ((size_t)(4 * TYc + 4) * g.px + (size_t)(2 * TYc + 1) * (g.px >> 1)) * sizeof(T);
+    dim3 grid((unsigned)(ntile * nseg * C.K));
+    launch(grid, threads, lds, C.stream, k_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, g, gc, F.st, TYc,
+           SZc, ntile, nseg);
+  }
+
   // f_{l+1} = R (f_l - A_l x_l) - B_{l+1} Sigma^{-1} B_l^T x_l
   void residual_restrict(int l) override {
     Level& F = *C.L[l];
     Level& Cc = *C.L[l + 1];
-    long long nf = F.N * C.K, nc = Cc.N * C.K;
+    if (rr_ok(l)) {
+      residual_restrict_fused(l);
+      if (C.m > 0) {
+        lr_dot_solve(l, X(l), C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+        lr_apply(f[l + 1].p, Cc.g.cs, Cc.Brows);
+      }
+      return;
+    }
+    dim3 gr = row_grid(F.g, F.g.px >> 3), gq = row_grid(Cc.g, (Cc.g.n0 + 3) / 4);
     if (F.g.dim == 3) {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
-      launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, F.g, F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, F.g, F.st);
+      launch(gq, 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g);
     } else {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, F.g, F.st, C.K);
-      launch(nblocks(nc), 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g, C.K);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, F.g, F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, F.g, F.st);
+      launch(gq, 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
     }
     if (C.m > 0) {
-      lr_dot(l, X(l));
-      lr_update(f[l + 1].p, Cc.g.cs, Cc.Brows, C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+      lr_dot_solve(l, X(l), C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+      lr_apply(f[l + 1].p, Cc.g.cs, Cc.Brows);
     }
+  }
+
+  // x_l += alpha P x_{l+1}; folded into the next sweep when that is a fused
+  // sweep of level l (the post-smoother follows the prolongation directly)
+  void prolongate_or_defer(int l, double alpha, bool sweep_follows) {
+    if (sweep_follows && C.tune_fold_prolong && fused_ok(l)) {
+      pending[l] = 1;
+      pending_alpha[l] = alpha;
+      return;
+    }
+    prolongate(l, alpha);
   }
 
   void prolongate(int l, double alpha) override {
     Level& F = *C.L[l];
     Level& Cc = *C.L[l + 1];
-    long long nf = F.N * C.K;
+    dim3 gr = row_grid(F.g, F.g.px >> 3);
     if (F.g.dim == 3)
-      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 3>, X(l), X(l + 1), F.g, Cc.g, (T)alpha, C.K);
+      launch(gr, 256, 0, C.stream, k_prolongate<T, 3>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
     else
-      launch(nblocks(nf), 256, 0, C.stream, k_prolongate<T, 2>, X(l), X(l + 1), F.g, Cc.g, (T)alpha, C.K);
+      launch(gr, 256, 0, C.stream, k_prolongate<T, 2>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
   }
 
   void coarse(int mode, const double* xi_dev) override {
@@ -435,14 +562,15 @@ struct Engine : EngineBase {
     size_t sh = 2 * (size_t)C.Nc * sizeof(double);
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
+    dim3 grid((unsigned)((C.Nc + 63) / 64), C.K);
     if (mode == NOISE_NONE)
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                           (int)C.Nc, xi_dev, nk);
     else if (mode == NOISE_PHILOX)
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                             (int)C.Nc, xi_dev, nk);
     else
-      launch(C.K, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
                                                               C.Qinv.p, (int)C.Nc, xi_dev, nk);
   }
 
@@ -465,7 +593,7 @@ struct Engine : EngineBase {
       residual_restrict(l);
       zero_x(l + 1);
       sample(l + 1);
-      prolongate(l, C.p.coarse_scaling);
+      prolongate_or_defer(l, C.p.coarse_scaling, C.p.npostsmooth > 0);
       post(l, true);
     }
   }
@@ -482,7 +610,7 @@ struct Engine : EngineBase {
       pre(l, false);
       residual_restrict(l);
       solve(l + 1);
-      prolongate(l, C.p.coarse_scaling);
+      prolongate_or_defer(l, C.p.coarse_scaling, C.p.npostsmooth > 0);
       post(l, false);
     }
   }
@@ -547,23 +675,24 @@ struct Engine : EngineBase {
   // r = b - A x - B Sigma^{-1} B^T x with x, b, r in padded FP64 layout
   void fine_residual_fp64(const double* x64, const double* b64, double* r64) override {
     Level& F = *C.L[0];
-    long long nf = F.N * C.K;
+    dim3 gr = row_grid(F.g, F.g.px >> 3);
     if (F.g.dim == 3) {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 3, true>, x64, b64, r64, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 3, false>, x64, b64, r64, F.g, F.st, C.K);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<double, 3, true>, x64, b64, r64, F.g, F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<double, 3, false>, x64, b64, r64, F.g, F.st);
     } else {
-      if (F.star) launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 2, true>, x64, b64, r64, F.g, F.st, C.K);
-      else launch(nblocks(nf), 256, 0, C.stream, k_residual<double, 2, false>, x64, b64, r64, F.g, F.st, C.K);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<double, 2, true>, x64, b64, r64, F.g, F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<double, 2, false>, x64, b64, r64, F.g, F.st);
     }
     if (C.m > 0) {
       dim3 grid(C.m, C.K);
-      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.z.p);
-      dim3 g2(nblocks(F.Brows.nrows), C.K);
       NoiseKey nk = key(0);
-      launch(g2, 256, 0, C.stream, k_lr_update<double, NOISE_NONE>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p,
-                                                                F.Brows.col.p, F.Brows.val.p, F.Brows.nrows,
-                                                                C.sigma_inv_mat.p, C.sqrt_sigma.p, C.z.p, -1.0, C.m,
-                                                                nullptr, nk);
+      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.z.p);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, C.sigma_inv_mat.p, C.sqrt_sigma.p, -1.0, nullptr,
+             C.u.p, C.m, nk);
+      dim3 g2(nblocks(F.Brows.nrows), C.K);
+      if (F.Brows.nrows)
+        launch(g2, 256, 0, C.stream, k_lr_update<double>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
+               F.Brows.val.p, F.Brows.nrows, C.u.p, C.m);
     }
   }
   void apply_neg_operator_fp64(const double* x64, double* y64) override { fine_residual_fp64(x64, nullptr, y64); }
@@ -694,14 +823,23 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     C.sqrt_sigma.upload(ss.data(), ss.size());
     C.sigma_inv_mat.upload(sim.data(), sim.size());
     C.z.alloc((size_t)C.m * C.K);
+    C.u.alloc((size_t)C.m * C.K);
   }
   C.Nc = C.H.coarse.Nc;
   if (C.Nc > 2048) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level has more than 2048 unknowns; add levels");
   {
     std::vector<long long> pi(C.H.coarse.pidx.begin(), C.H.coarse.pidx.end());
     C.cpidx.upload(pi.data(), pi.size());
-    C.LinvT.upload(C.H.coarse.LinvT.data(), C.H.coarse.LinvT.size());
-    C.Qinv.upload(C.H.coarse.Qinv.data(), C.H.coarse.Qinv.size());
+    // column-major copies for k_coarse
+    const int64_t Nc = C.H.coarse.Nc;
+    std::vector<double> lt((size_t)Nc * Nc), q((size_t)Nc * Nc);
+    for (int64_t r = 0; r < Nc; ++r)
+      for (int64_t j = 0; j < Nc; ++j) {
+        lt[(size_t)j * Nc + r] = C.H.coarse.LinvT[(size_t)r * Nc + j];
+        q[(size_t)j * Nc + r] = C.H.coarse.Qinv[(size_t)r * Nc + j];
+      }
+    C.LinvT.upload(lt.data(), lt.size());
+    C.Qinv.upload(q.data(), q.size());
   }
   uint32_t c0[2] = {0, 0};
   C.ctr.upload(c0, 2);
@@ -1040,6 +1178,15 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
   if (!h || !key) throw Error(MGMC_ERR_ARG, "null argument");
   std::string k(key);
   if (k == "fused_sweep") h->C.tune_fused = value != 0;
+  else if (k == "plane_sweep") h->C.tune_planes = value != 0;
+  else if (k == "fold_prolongation") h->C.tune_fold_prolong = value != 0;
+  else if (k == "fused_residual") h->C.tune_fused_rr = value != 0;
+  else if (k == "rr_threads") {
+    if (value < 64 || value > 1024) throw Error(MGMC_ERR_ARG, "rr_threads must lie in [64, 1024]");
+    h->C.tune_rr_threads = value;
+  }
+  else if (k == "plane_sweep_min") h->C.tune_planes_min = value;
+  else if (k == "plane_sweep_stage3") h->C.tune_planes_stage3 = value != 0;
   else if (k == "rb_threads") {
     if (value < 64 || value > 1024) throw Error(MGMC_ERR_ARG, "rb_threads must lie in [64, 1024]");
     h->C.tune_rb_threads = value;

@@ -203,3 +203,72 @@ def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
                 a = ref.sweep(0, fw, st, b, x, **kw)
                 c = fus.sweep(0, fw, st, b, x, **kw)
                 assert np.abs(a - c).max() <= tol * np.abs(a).max(), (fw, st, bool(kw))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+@pytest.mark.parametrize("stage3", [0, 1])
+def test_plane_sweep_matches_colour_phases(precision, tol, stage3):
+    """Box-stencil levels swept in two plane-parity launches equal the eight
+    colour-phase launches (Philox and injected noise, both directions)."""
+    op, _ = make_problem(3, (32, 32, 16), 3)
+    p = params(3)
+    K = 2
+    ref = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    ref.set_tuning("plane_sweep", 0)
+    new = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    new.set_tuning("plane_sweep_min", 1)
+    new.set_tuning("plane_sweep_stage3", stage3)
+    rng = np.random.default_rng(10)
+    for level in (1, 2):
+        N = ref.ndof(level)
+        b, x, xi = rng.standard_normal((3, K, N))
+        xl = rng.standard_normal((K, 3))
+        for fw in (True, False):
+            for st, kw in ((False, {}), (True, {}), (True, {"xi": xi, "xi_lr": xl})):
+                a = ref.sweep(level, fw, st, b, x, **kw)
+                c = new.sweep(level, fw, st, b, x, **kw)
+                assert np.abs(a - c).max() <= tol * np.abs(a).max(), (level, fw, st, bool(kw))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("smoother", ["SSOR", "SOR"])
+def test_folded_prolongation_matches_separate(precision, tol, smoother):
+    """The coarse-grid correction added while the first post-sweep reads its
+    planes equals a separate prolongate-add launch (whole MGMC cycles)."""
+    op, _ = make_problem(3, (32, 32, 32), 4)
+    p = params(4, smoother=smoother)
+    N = op.lattice.Nvertex
+    f = np.random.default_rng(12).standard_normal(N)
+    out = []
+    for fold in (0, 1):
+        s = mg.MultigridMCSampler(op, p, nchains=2, precision=precision, seed=21)
+        s.set_tuning("fold_prolongation", fold)
+        x = np.zeros((2, N))
+        for _ in range(3):
+            s.apply(f, x)
+        out.append(x)
+    assert np.abs(out[0] - out[1]).max() <= tol * np.abs(out[0]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,threads", [((16, 24, 32), 64), ((16, 24, 32), 512), ((40, 24, 16), 128)])
+def test_fused_residual_restriction_matches_oracle(n, threads):
+    """The single-pass residual + restriction kernel of 3D star levels equals
+    R(f - Q x) of the oracle for tiles of a few coarse rows up to whole planes."""
+    op, ora = make_problem(3, n, 3)
+    p = params(3)
+    K = 3
+    omg = oracle_mg(ora, p)
+    rng = np.random.default_rng(31)
+    s = mg.MultigridMCSampler(op, p, nchains=K)
+    s.set_tuning("rr_threads", threads)
+    ref = mg.MultigridMCSampler(op, p, nchains=K)
+    ref.set_tuning("fused_residual", 0)
+    N = s.ndof(0)
+    f, x = rng.standard_normal((K, N)), rng.standard_normal((K, N))
+    got = s.residual_restrict(0, f, x)
+    sep = ref.residual_restrict(0, f, x)
+    for c in range(K):
+        want = omg.restrict(0, f[c] - omg.level_apply(0, x[c]))
+        assert _rel(got[c], want) < 1e-12
+        assert _rel(got[c], sep[c]) < 1e-13
