@@ -147,13 +147,19 @@ def main():
     s32.run(10, record=False)
     ms_sweep, nlaunch = s32.profile_result()
     s32.profile(0)
+    # algorithmic bytes: interior vertices only; f is one vector shared by the
+    # chains (read once per launch)
     w = 4
     if args.colour_phases:
         kernel = "k_sweep<float,3,star,philox> (level-0 red-black colour phase)"
-        bytes_per_launch = 2 * w * N * K  # read x (all), read f + write x (one colour)
+        bytes_per_launch = w * N * K + w * N * K // 2 + w * N  # read x (all), write x (one colour), f
     else:
         kernel = "k_sweep_rb3<float,philox> (level-0 red-black sweep, one pass)"
-        bytes_per_launch = 3 * w * N * K  # read x, read f, write x
+        # read x, write x, f once; the first post-smoothing launch of the cycle
+        # also reads the level-1 correction (one of the 2 (npre + npost) launches)
+        Nc = (n // 2 - 1) ** 3
+        nlaunch_cycle = 2 * (params.npresmooth + params.npostsmooth)
+        bytes_per_launch = 2 * w * N * K + w * N + w * Nc * K / nlaunch_cycle
     achieved = bytes_per_launch / (ms_sweep * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", ("sweep_l0_fp32_128cube_k64_traffic.json" if args.colour_phases

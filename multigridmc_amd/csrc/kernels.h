@@ -20,7 +20,8 @@ namespace mgmc {
 struct DevGeom {
   int dim, n0, n1, n2;
   int px, py, pz;
-  long long cs;  // chain stride
+  long long cs;   // chain stride
+  long long fcs;  // chain stride of the right-hand side f (0: one f shared by all chains)
 };
 
 // Constant stencil (27 entries, (dz+1)*9 + (dy+1)*3 + (dx+1)) and the
@@ -32,6 +33,15 @@ struct Stencil {
   double omega;
   double inv_diag_omega;  // omega / D
   double noise_scale;     // sqrt(D (2 - omega) / omega)
+};
+
+// The stencil and SOR constants in the field precision, for kernels whose
+// coefficients then stay in scalar registers (no per-use conversion).
+template <typename T>
+struct StencilT {
+  T c[27];
+  T inv_diag_omega;
+  T noise_scale;
 };
 
 // Philox key and the noise tag of a launch; the sample counter is read from

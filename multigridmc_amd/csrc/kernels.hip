@@ -91,9 +91,9 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
         w[0] = p[-1];
         w[9] = p[8];
       }
-      const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
-      const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
-      const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
+      const T a0 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
+      const T a1 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
+      const T a2 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         const int q = 2 * l + IPAR;
@@ -101,15 +101,15 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
       }
     }
   }
-  const T om_d = (T)st.inv_diag_omega;
+  const T om_d = st.inv_diag_omega;
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const int q = 2 * l + IPAR;
     const int i = i0 + q;
     if (i < 1 || i > g.n0 - 1) continue;
     T b = fb[q];
-    if (NOISE == NOISE_PHILOX) b += (T)st.noise_scale * (T)xi[l];
-    if (NOISE == NOISE_EXTERNAL) b += (T)st.noise_scale * extg[q];
+    if (NOISE == NOISE_PHILOX) b += st.noise_scale * (T)xi[l];
+    if (NOISE == NOISE_EXTERNAL) b += st.noise_scale * extg[q];
     xc[q + 1] += om_d * (b - acc[l]);
   }
 }
@@ -120,7 +120,7 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
 // global memory). Outer margins of the first / last group are never used.
 template <int IPAR, typename T, int NOISE>
 __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* zm, const T* zp, const DevGeom& g,
-                                                  const Stencil& st, const T* fb, const float* xi, const T* extg,
+                                                  const StencilT<T>& st, const T* fb, const float* xi, const T* extg,
                                                   int i0, int gx, int ngx) {
   T acc[4];
 #pragma unroll
@@ -139,9 +139,9 @@ __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* 
         w[0] = gx > 0 ? p[-1] : (T)0;
         w[9] = gx < ngx - 1 ? p[8] : (T)0;
       }
-      const T a0 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
-      const T a1 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
-      const T a2 = (T)st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
+      const T a0 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
+      const T a1 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
+      const T a2 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         const int q = 2 * l + IPAR;
@@ -149,41 +149,89 @@ __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* 
       }
     }
   }
-  const T om_d = (T)st.inv_diag_omega;
+  const T om_d = st.inv_diag_omega;
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const int q = 2 * l + IPAR;
     const int i = i0 + q;
     if (i < 1 || i > g.n0 - 1) continue;
     T b = fb[q];
-    if (NOISE == NOISE_PHILOX) b += (T)st.noise_scale * (T)xi[l];
-    if (NOISE == NOISE_EXTERNAL) b += (T)st.noise_scale * extg[q];
+    if (NOISE == NOISE_PHILOX) b += st.noise_scale * (T)xi[l];
+    if (NOISE == NOISE_EXTERNAL) b += st.noise_scale * extg[q];
     xc[q + 1] += om_d * (b - acc[l]);
   }
 }
 
-// Adds alpha * (P xc) to the fine 8-vertex group (8 gx .. 8 gx + 7, row j,
-// plane k) held in v: the linear interpolation of the coarse field xc
-// (chain base pointer cb, geometry gc) as in k_prolongate. Fine boundary
-// vertices receive only coarse boundary values, which are zero.
-template <typename T, int DIM>
-__device__ __forceinline__ void add_prolongation8(T* v, const T* cb, const DevGeom& gc, int gx, int j, int k,
-                                                  T alpha) {
-  const T half = (T)0.5;
-  const int j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
-  const int nz = (DIM == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1;
-  T s[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};
-  for (int a = 0; a < nz; ++a)
-    for (int b2 = 0; b2 < ny; ++b2) {
-      const int K = a ? k1 : k0, J = b2 ? j1 : j0;
-      const T* p = cb + ((long long)K * gc.py + J) * gc.px + 4 * gx;
+// x += alpha P xc for the fine 8-vertex group (8 gx .. 8 gx + 7, row j) of a
+// thread that visits the fine planes of its row in increasing order (the
+// gather form of k_prolongate, fused into the loads of a sweep). Fine
+// boundary vertices receive only coarse boundary values, which are zero.
+// The thread holds the y-interpolated
+// coarse values s(K) and s(K+1) of the two coarse planes around the current
+// fine plane, so every coarse plane is read once per thread (two aligned
+// 4-vertex loads and one single load per coarse row).
+template <typename T>
+struct ProlongStream {
+  const T* cb;  // coarse chain base, nullptr: no correction
+  DevGeom gc;
+  int gx, j0, j1, kl;
+  T alpha;
+  T sl[5], sh[5];
+
+  __device__ __forceinline__ void row_values(int K, T* s) const {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) s[e] += (4 * gx + e < gc.px) ? p[e] : (T)0;
+    for (int e = 0; e < 5; ++e) s[e] = (T)0;
+    if (K > gc.n2) return;  // beyond the last coarse plane: boundary, zero
+    const int nr = j1 != j0 ? 2 : 1;
+    for (int b = 0; b < nr; ++b) {
+      const T* p = cb + ((long long)K * gc.py + (b ? j1 : j0)) * gc.px + 4 * gx;
+      T v[4];
+      if constexpr (sizeof(T) == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      } else {
+        const double2 a = reinterpret_cast<const double2*>(p)[0], c = reinterpret_cast<const double2*>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = c.x; v[3] = c.y;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) s[e] += v[e];
+      s[4] += 4 * gx + 4 < gc.px ? p[4] : (T)0;
     }
-  const T w = (nz == 2 ? half : (T)1) * (ny == 2 ? half : (T)1) * alpha;
+    if (nr == 2) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] += w * ((q & 1) ? half * (s[q >> 1] + s[(q >> 1) + 1]) : s[q >> 1]);
-}
+      for (int e = 0; e < 5; ++e) s[e] *= (T)0.5;
+    }
+  }
+  // first fine plane this thread will visit
+  __device__ __forceinline__ void init(const T* coarse, const DevGeom& g_c, int gx_, int j, int p0, T a) {
+    cb = coarse;
+    gc = g_c;
+    gx = gx_;
+    j0 = j >> 1;
+    j1 = (j + 1) >> 1;
+    alpha = a;
+    kl = p0 >> 1;
+    if (!cb) return;
+    row_values(kl, sl);
+    row_values(kl + 1, sh);
+  }
+  // v += alpha (P xc) on fine plane p (p never below the previous call's)
+  __device__ __forceinline__ void add(T* v, int p) {
+    if (!cb) return;
+    if ((p >> 1) != kl) {
+      ++kl;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) sl[e] = sh[e];
+      row_values(kl + 1, sh);
+    }
+    T t[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) t[e] = (p & 1) ? (T)0.5 * (sl[e] + sh[e]) : sl[e];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] += alpha * ((q & 1) ? (T)0.5 * (t[q >> 1] + t[(q >> 1) + 1]) : t[q >> 1]);
+  }
+};
+
 
 // ---------------------------------------------------------------------------
 // One colour phase of an SOR sweep (SORSmoother::apply_sparse,
@@ -200,7 +248,8 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int gx = (int)(t % ngx);
   long long row = t / ngx;
-  if (row >= (long long)nrows * K) return;
+  // groups holding no interior vertex (the x padding) change nothing
+  if (row >= (long long)nrows * K || 8 * gx > g.n0 - 1) return;
   int chain = (int)(row / nrows);
   int rr = (int)(row % nrows);
   int j, k, ipar;
@@ -231,7 +280,7 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
   xc[0] = xr[-1];
   xc[9] = xr[8];
   T fb[8];
-  load8<T>(f + base, fb);
+  load8<T>(f + chain * g.fcs + rowoff + 8 * gx, fb);
 
   float xi[4] = {0.f, 0.f, 0.f, 0.f};
   if (NOISE == NOISE_PHILOX) {
@@ -283,20 +332,39 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
 // Residual:   M = Sigma^{-1}, a = 0, zc = -1 -> subtract B Sigma^{-1} B^T x.
 // ---------------------------------------------------------------------------
 constexpr int MMAX = 128;
+// z[c][k] = (B^T x_c)_k for the LR_CPB chains c0 .. c0 + LR_CPB - 1 of a
+// block (grid: columns x chain groups): the column's indices and values are
+// read once and the chains' x loads are independent.
+constexpr int LR_CPB = 1;
 template <typename T>
 __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
                                                 const long long* __restrict__ bidx, const double* __restrict__ bval,
-                                                int m, double* __restrict__ z) {
-  const int kcol = blockIdx.x, c = blockIdx.y;
-  const T* xc = x + c * cs;
-  double s = 0.0;
-  for (int e = bptr[kcol] + threadIdx.x; e < bptr[kcol + 1]; e += blockDim.x) s += bval[e] * (double)xc[bidx[e]];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  __shared__ double red[4];
+                                                int m, int K, double* __restrict__ z) {
+  const int kcol = blockIdx.x, c0 = blockIdx.y * LR_CPB;
+  const int nc = K - c0 < LR_CPB ? K - c0 : LR_CPB;
+  double s[LR_CPB];
+#pragma unroll
+  for (int q = 0; q < LR_CPB; ++q) s[q] = 0.0;
+  for (int e = bptr[kcol] + threadIdx.x; e < bptr[kcol + 1]; e += blockDim.x) {
+    const double b = bval[e];
+    const T* xp = x + (long long)c0 * cs + bidx[e];
+#pragma unroll
+    for (int q = 0; q < LR_CPB; ++q)
+      if (q < nc) s[q] += b * (double)xp[q * cs];
+  }
+  __shared__ double red[LR_CPB][4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = s;
+#pragma unroll
+  for (int q = 0; q < LR_CPB; ++q) {
+    double v = s[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[q][wid] = v;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) z[(long long)c * m + kcol] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x < nc) {
+    const int q = threadIdx.x;
+    z[(long long)(c0 + q) * m + kcol] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+  }
 }
 
 // one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c)
@@ -325,22 +393,35 @@ __global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z,
   }
 }
 
+// x_c[rows[r]] += (R u_c)_r for the LR_CPU chains of a block (grid: row
+// blocks x chain groups); a row's entries are read once for all its chains.
+constexpr int LR_CPU = 8;
 template <typename T>
 __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
                                                    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                    const double* __restrict__ val, int nrows,
-                                                   const double* __restrict__ u, int m) {
-  __shared__ double us[MMAX];
-  const int c = blockIdx.y;
+                                                   const double* __restrict__ u, int m, int K) {
+  __shared__ double us[LR_CPU][MMAX];
+  const int c0 = blockIdx.y * LR_CPU;
+  const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
   const int tid = threadIdx.x;
-  if (tid < m) us[tid] = u[(long long)c * m + tid];
+  for (int e = tid; e < nc * m; e += blockDim.x) us[e / m][e % m] = u[(long long)c0 * m + e];
   __syncthreads();
   const int r = blockIdx.x * blockDim.x + tid;
   if (r >= nrows) return;
-  double s = 0.0;
-  for (int e = ptr[r]; e < ptr[r + 1]; ++e) s += val[e] * us[col[e]];
-  T* p = x + c * cs + rows[r];
-  *p = (T)((double)*p + s);
+  double s[LR_CPU];
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q) s[q] = 0.0;
+  for (int e = ptr[r]; e < ptr[r + 1]; ++e) {
+    const double v = val[e];
+    const int cl = col[e];
+#pragma unroll
+    for (int q = 0; q < LR_CPU; ++q) s[q] += v * us[q][cl];
+  }
+  T* p = x + (long long)c0 * cs + rows[r];
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q)
+    if (q < nc) p[q * cs] = (T)((double)p[q * cs] + s[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +486,7 @@ __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const
       }
   }
   T fb[8], out[8];
-  if (f) load8<T>(f + base, fb);
+  if (f) load8<T>(f + (long long)c * g.fcs + ((long long)k * g.py + j) * g.px + 8 * gx, fb);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int i = 8 * gx + q;
@@ -544,7 +625,7 @@ __global__ void __launch_bounds__(256) k_unpack(const T* __restrict__ pad, doubl
 // through LDS. The matrices are stored column-major (element (r, j) at
 // j * Nc + r) so that a wave reads 64 consecutive rows of one column.
 template <typename T, int NOISE>
-__global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs,
+__global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
                                                 const long long* __restrict__ pidx, const double* __restrict__ LT,
                                                 const double* __restrict__ Q, int Nc, const double* __restrict__ ext,
                                                 NoiseKey nk) {
@@ -555,7 +636,7 @@ __global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __re
   const int c = blockIdx.y;
   const int tid = threadIdx.x;
   for (int j = tid; j < Nc; j += blockDim.x) {
-    fs[j] = (double)f[c * cs + pidx[j]];
+    fs[j] = (double)f[c * fcs + pidx[j]];
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
       normals4(COARSE_SLOT + (uint32_t)(j >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
@@ -604,9 +685,17 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
                                                     NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
                                                     T alpha) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
-  const size_t asz = (size_t)(TY + 4) * g.px, bsz = (size_t)(TY + 2) * g.px;
-  T* A0 = reinterpret_cast<T*>(lds_raw);
+  // LDS: [guard][A (k even)][guard][A (k odd)][guard][B (k even)][guard][B (k odd)][guard];
+  // the zero guards stand for the x neighbours beyond the first and last row
+  constexpr int LG = 8;
+  const size_t asz = (size_t)(TY + 4) * g.px + LG, bsz = (size_t)(TY + 2) * g.px + LG;
+  T* A0 = reinterpret_cast<T*>(lds_raw) + LG;
   T* B0 = A0 + 2 * asz;
+  if (threadIdx.x < 5 * LG) {
+    const int e = threadIdx.x / LG, o = threadIdx.x % LG;
+    T* gptr = e < 3 ? A0 - LG + e * asz : B0 - LG + (e - 2) * bsz;
+    gptr[o] = (T)0;
+  }
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int r = tid / ngx, gx = tid % ngx;
@@ -625,6 +714,7 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
   const int ke = ks + SZ < g.n2 ? ks + SZ : g.n2;  // owned interior planes [ks, ke)
   const long long pl = (long long)g.px * g.py;
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+  const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
   const uint32_t sample = *nk.sample;
   const int second = 1 - first;
 
@@ -632,46 +722,49 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
 #pragma unroll
   for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = (T)0;
   // optional prolongate-add of the coarse correction while the planes are read
-  const T* cb = xcoarse ? xcoarse + (long long)chain * gc.cs : nullptr;
+  ProlongStream<T> pro;
+  pro.init(xcoarse && has_mem ? xcoarse + (long long)chain * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1,
+           alpha);
   if (has_mem && ks - 2 >= 0) {
     load8<T>(x + cbase + (ks - 2) * pl, xm);
-    if (cb) add_prolongation8<T, 3>(xm, cb, gc, gx, j, ks - 2, alpha);
+    pro.add(xm, ks - 2);
   }
   if (has_mem) {
     load8<T>(x + cbase + (ks - 1) * pl, xc);
-    if (cb) add_prolongation8<T, 3>(xc, cb, gc, gx, j, ks - 1, alpha);
+    pro.add(xc, ks - 1);
   }
+  // groups holding no interior vertex (the x padding) draw no noise and update nothing
+  const bool inner = 8 * gx <= g.n0 - 1;
 
   const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
   const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
   const size_t aoff = (size_t)r * g.px + 8 * gx;        // A rows j0-2 .. j0+TY+1
   const size_t boff = (size_t)(r - 1) * g.px + 8 * gx;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
 
+  // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
+  // receives plane k+1; fm, fc = f on planes k-1, k
   for (int k = ks - 1; k <= ke; ++k) {
     const int kb = k - 1;
-    const bool doA = ring && k >= 1 && k <= g.n2 - 1;
-    const bool doB = owned && kb >= ks && kb <= ke - 1;
+    const bool doA = ring && inner && k >= 1 && k <= g.n2 - 1;
+    const bool storeB = owned && kb >= ks && kb <= ke - 1;
+    const bool doB = storeB && inner;
     // loads first: next plane of x, f on plane k
     if (has_mem && k + 1 <= g.n2) {
       load8<T>(x + cbase + (long long)(k + 1) * pl, xp);
-      if (cb) add_prolongation8<T, 3>(xp, cb, gc, gx, j, k + 1, alpha);
+      pro.add(xp, k + 1);
     } else {
 #pragma unroll
       for (int q = 0; q < 8; ++q) xp[q] = (T)0;
     }
-    if (doA) load8<T>(f + cbase + (long long)k * pl, fc);
+    if (doA) load8<T>(f + fbase + (long long)k * pl, fc);
     // noise of both colours while the loads are in flight
     const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
     float xiA[4] = {0.f, 0.f, 0.f, 0.f}, xiB[4] = {0.f, 0.f, 0.f, 0.f};
     if (NOISE == NOISE_PHILOX) {
-      if (doA) {
-        uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + gx) << 1) | (uint32_t)iparA;
-        normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
-      }
-      if (doB) {
-        uint32_t c0 = (uint32_t)((((long long)kb * g.py + j) * ngx + gx) << 1) | (uint32_t)iparB;
-        normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
-      }
+      const uint32_t cj = (uint32_t)(j * ngx + gx);
+      const uint32_t cpl = (uint32_t)(g.py * ngx);
+      if (doA) normals4(((k * cpl + cj) << 1) | (uint32_t)iparA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
+      if (doB) normals4(((kb * cpl + cj) << 1) | (uint32_t)iparB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
     }
     T* A = A0 + (k & 1) * asz;         // plane k before the first colour
     const T* Bm = B0 + (kb & 1) * bsz;  // plane k-1 after the first colour (written last step)
@@ -705,8 +798,8 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
         star_update4<0, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
       else
         star_update4<1, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
-      store8<T>(xout + cbase + (long long)kb * pl, xm);
     }
+    if (storeB) store8<T>(xout + cbase + (long long)kb * pl, xm);
     // plane k after the first colour, for the second colour at the next step
     if (active && r >= 1 && r <= TY + 2) store8<T>(Bk + boff, xc);
 #pragma unroll
@@ -726,20 +819,24 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
 // k +- 1 only), so one workgroup per (plane, chain) reproduces the colour-
 // phase launches of k_sweep for colours 4 kpar .. 4 kpar + 3 (forward) or the
 // reverse (backward). Plane k lives in LDS; planes k +- 1 are read-only here.
-// One thread per aligned 8-vertex group of a row.
+// One thread per aligned 8-vertex group of a row; the first `hth` threads
+// (a multiple of 64) hold the odd rows and the rest the even rows, so that
+// the waves of the row parity a colour phase does not touch skip it whole.
 // ---------------------------------------------------------------------------
 template <typename T, int NOISE, bool STAGE3>
 __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
-                                                           const T* __restrict__ ext, DevGeom g, Stencil st,
-                                                           int kpar, int forward, int nplanes, NoiseKey nk) {
+                                                           const T* __restrict__ ext, DevGeom g, StencilT<T> st,
+                                                           int kpar, int forward, int nplanes, int hth, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
   // STAGE3: planes k-1, k, k+1 in LDS (py x px each); else plane k only
   const size_t psz = (size_t)g.py * g.px;
   T* P = reinterpret_cast<T*>(lds_raw) + (STAGE3 ? psz : 0);
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
-  const int r = tid / ngx, gx = tid % ngx;
-  const int j = r + 1;
+  const bool odd = tid < hth;
+  const int t2 = odd ? tid : tid - hth;
+  const int gx = t2 % ngx;
+  const int j = 2 * (t2 / ngx) + (odd ? 1 : 2);
   const bool active = j <= g.n1 - 1;
   const int pidx = blockIdx.x % nplanes;
   const int chain = blockIdx.x / nplanes;
@@ -754,14 +851,14 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
       store8<T>(P + dz * (long long)psz + (size_t)rr * g.px + 8 * gg, v);
     }
   T fb[8];
-  if (active) load8<T>(f + base, fb);
+  if (active) load8<T>(f + base + (long long)chain * (g.fcs - g.cs), fb);
   const uint32_t sample = *nk.sample;
   const int jpar = j & 1;
   const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
   for (int ph = 0; ph < 4; ++ph) {
     const int q = forward ? ph : 3 - ph;  // in-plane colour: ipar = q & 1, jpar = q >> 1
     __syncthreads();
-    if (!active || jpar != (q >> 1)) continue;
+    if (!active || jpar != (q >> 1) || 8 * gx > g.n0 - 1) continue;
     const int ipar = q & 1;
     T* row = P + (size_t)j * g.px + 8 * gx;
     T xc[10];
@@ -829,25 +926,28 @@ __global__ void __launch_bounds__(1024) k_residual_restrict_star3(const T* __res
   const int Ke = Ks + SZc < gc.n2 ? Ks + SZc : gc.n2;  // coarse planes [Ks, Ke)
   const long long pl = (long long)g.px * g.py;
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+  const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
   const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
   const T half = (T)0.5;
 
-  T xm[8], xc[8], xp[8], fb[8];
+  T xm[8], xc[8], xp[8], fb[8], fn[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = fb[q] = (T)0;
+  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = fb[q] = fn[q] = (T)0;
   const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;
   if (has_mem) {
     load8<T>(x + cbase + (long long)(p0 - 1) * pl, xm);
     load8<T>(x + cbase + (long long)p0 * pl, xc);
   }
+  if (rvalid) load8<T>(f + fbase + (long long)p0 * pl, fb);
   T acc[4] = {(T)0, (T)0, (T)0, (T)0};
   T* LXrow = LX + (size_t)rt * g.px + 8 * gx;
   T* LRrow = LR + (size_t)(rt - 1) * g.px + 8 * gx;
   T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
 
   for (int p = p0; p <= p1; ++p) {
+    // planes p + 1 of x and f are in flight while plane p is processed
     if (has_mem) load8<T>(x + cbase + (long long)(p + 1) * pl, xp);
-    if (rvalid) load8<T>(f + cbase + (long long)p * pl, fb);
+    if (rvalid && p < p1) load8<T>(f + fbase + (long long)(p + 1) * pl, fn);
     if (active) store8<T>(LXrow, xc);
     __syncthreads();
     T r[8];
@@ -905,6 +1005,7 @@ __global__ void __launch_bounds__(1024) k_residual_restrict_star3(const T* __res
     for (int q = 0; q < 8; ++q) {
       xm[q] = xc[q];
       xc[q] = xp[q];
+      fb[q] = fn[q];
     }
   }
 }

@@ -158,6 +158,7 @@ struct EngineBase {
   virtual void set_f_from_fp64(const double* pad64) = 0;
   virtual void add_x_to_fp64(double* pad64) = 0;
   virtual void retune() = 0;
+  virtual bool f_shared() const = 0;
 };
 
 // A level field for all chains with a zeroed guard of GUARD elements on both
@@ -189,6 +190,16 @@ struct Engine : EngineBase {
   Field<T> ext;  // external noise staging (test hooks)
 
   T* X(int l) { return cur[l] ? x2[l].p : x[l].p; }
+  // Level geometry as the kernels see it: when every chain was given the same
+  // right-hand side on level 0 (the sampler's f), the level-0 kernels read
+  // chain 0's copy for all chains (fcs = 0), so f costs one read per level
+  // pass instead of one per chain. All K copies are still written.
+  bool f0_shared = false;
+  DevGeom geo(int l) const {
+    DevGeom g = C.L[l]->g;
+    g.fcs = (l == 0 && f0_shared) ? 0 : g.cs;
+    return g;
+  }
 
   explicit Engine(Common& c) : C(c) {
     x.resize(C.nlevel);
@@ -218,7 +229,7 @@ struct Engine : EngineBase {
     const DevGeom& g = C.L[l]->g;
     int ngx = g.px >> 3;
     int by_threads = C.tune_rb_threads / ngx - 4;
-    int by_lds = (int)((65536 / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
+    int by_lds = (int)(((65536 - 40 * sizeof(T)) / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
     return std::min(std::min(by_threads, by_lds), g.n1 - 1);
   }
   bool fused_ok(int l) const {
@@ -238,7 +249,7 @@ struct Engine : EngineBase {
     s.SZ = (nplanes + nseg - 1) / nseg;
     s.nseg = (nplanes + s.SZ - 1) / s.SZ;
     s.threads = ((ngx * (s.TY + 4) + 63) / 64) * 64;
-    s.lds = (size_t)2 * (2 * s.TY + 6) * g.px * sizeof(T);
+    s.lds = ((size_t)2 * (2 * s.TY + 6) * g.px + 5 * 8) * sizeof(T);  // + the zero guards of k_sweep_rb3
     return s;
   }
   // prolongate-add pending for the next fused sweep of a level (folded into its loads)
@@ -264,13 +275,13 @@ struct Engine : EngineBase {
       pending[l] = 0;
     }
     if (noise == NOISE_NONE)
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, Lv.g, Lv.st, first,
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, geo(l), Lv.st, first,
              s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     else if (noise == NOISE_PHILOX)
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_PHILOX>, in, out, f[l].p, extp, Lv.g, Lv.st,
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_PHILOX>, in, out, f[l].p, extp, geo(l), Lv.st,
              first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     else
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_EXTERNAL>, in, out, f[l].p, extp, Lv.g, Lv.st,
+      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_EXTERNAL>, in, out, f[l].p, extp, geo(l), Lv.st,
              first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
     if (timed) prof_end();
     cur[l] ^= 1;
@@ -281,18 +292,29 @@ struct Engine : EngineBase {
     HIP_CHECK(hipStreamSynchronize(C.stream));
   }
   // ---- plane-parity sweep of 3D box levels (k_sweep_box_planes) ----
+  // threads of one row-parity half of a plane workgroup (odd rows j = 1, 3, ..)
+  static int plane_half_threads(const DevGeom& g) { return ((g.n1 / 2) * (g.px >> 3) + 63) / 64 * 64; }
+  static StencilT<T> stencil_t(const Stencil& s) {
+    StencilT<T> t;
+    for (int e = 0; e < 27; ++e) t.c[e] = (T)s.c[e];
+    t.inv_diag_omega = (T)s.inv_diag_omega;
+    t.noise_scale = (T)s.noise_scale;
+    return t;
+  }
   bool planes_ok(int l) const {
     const Level& Lv = *C.L[l];
     if (!C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
     const DevGeom& g = Lv.g;
     // one workgroup per (plane, chain): worth it only with enough workgroups
-    return (g.n1 - 1) * (g.px >> 3) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= 65536 &&
+    return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= 65536 &&
            (long long)((g.n2 - 1) / 2) * C.K >= C.tune_planes_min;
   }
   void plane_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
     Level& Lv = *C.L[l];
     const DevGeom& g = Lv.g;
-    const int threads = (((g.n1 - 1) * (g.px >> 3) + 63) / 64) * 64;
+    const int hth = plane_half_threads(g);
+    const int threads = 2 * hth;
+    const StencilT<T> stt = stencil_t(Lv.st);
     const size_t plane = (size_t)g.py * g.px * sizeof(T);
     const bool stage3 = C.tune_planes_stage3 && 3 * plane <= 65536;
     const size_t lds = stage3 ? 3 * plane : plane;
@@ -303,8 +325,8 @@ struct Engine : EngineBase {
       if (nplanes <= 0) continue;
       dim3 grid((unsigned)(nplanes * C.K));
 #define MGMC_PLANES(NOISE, S3)                                                                                  \
-  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, g, Lv.st, kpar, \
-         (int)forward, nplanes, nk)
+  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, geo(l), stt, kpar, \
+         (int)forward, nplanes, hth, nk)
       if (stage3) {
         if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, true);
         else if (noise == NOISE_PHILOX) MGMC_PLANES(NOISE_PHILOX, true);
@@ -375,7 +397,8 @@ struct Engine : EngineBase {
     NoiseKey nk = key(tag);
     const bool timed = prof_begin(l);
 #define MGMC_SWEEP(DIM, STAR, NOISE) \
-  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, g, Lv.st, colour, nrows, C.K, nk)
+  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, geo(l), Lv.st, colour, nrows, C.K, \
+         nk)
 #define MGMC_SWEEP_N(DIM, STAR)                                 \
   do {                                                          \
     if (noise == NOISE_NONE) MGMC_SWEEP(DIM, STAR, NOISE_NONE); \
@@ -397,8 +420,8 @@ struct Engine : EngineBase {
   // u = M (a sqrt(Sigma) xi' - B_l^T x) for every chain (k_lr_dot + k_lr_solve)
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
-    dim3 grid(C.m, C.K);
-    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.z.p);
+    dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
+    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p);
     NoiseKey nk = key(tag);
     if (noise == NOISE_NONE)
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
@@ -412,8 +435,9 @@ struct Engine : EngineBase {
   // x[rows(R)] += R u
   void lr_apply(T* xp, long long cs, DevRows& R) {
     if (R.nrows == 0) return;
-    dim3 grid(nblocks(R.nrows), C.K);
-    launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m);
+    dim3 grid(nblocks(R.nrows), (C.K + LR_CPU - 1) / LR_CPU);
+    launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
+           C.K);
   }
 
   // One directional SOR sweep (+ SMW), SORSmoother::apply / SORSampler::apply
@@ -503,7 +527,7 @@ struct Engine : EngineBase {
     const int threads = ((ngx * (2 * TYc + 3) + 63) / 64) * 64;
     const size_t lds = ((size_t)(4 * TYc + 4) * g.px + (size_t)(2 * TYc + 1) * (g.px >> 1)) * sizeof(T);
     dim3 grid((unsigned)(ntile * nseg * C.K));
-    launch(grid, threads, lds, C.stream, k_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, g, gc, F.st, TYc,
+    launch(grid, threads, lds, C.stream, k_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, geo(l), gc, F.st, TYc,
            SZc, ntile, nseg);
   }
 
@@ -521,12 +545,12 @@ struct Engine : EngineBase {
     }
     dim3 gr = row_grid(F.g, F.g.px >> 3), gq = row_grid(Cc.g, (Cc.g.n0 + 3) / 4);
     if (F.g.dim == 3) {
-      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, F.g, F.st);
-      else launch(gr, 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, F.g, F.st);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
       launch(gq, 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g);
     } else {
-      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, F.g, F.st);
-      else launch(gr, 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, F.g, F.st);
+      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      else launch(gr, 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
       launch(gq, 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
     }
     if (C.m > 0) {
@@ -564,13 +588,13 @@ struct Engine : EngineBase {
     NoiseKey nk = key(tag);
     dim3 grid((unsigned)((C.Nc + 63) / 64), C.K);
     if (mode == NOISE_NONE)
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                           (int)C.Nc, xi_dev, nk);
     else if (mode == NOISE_PHILOX)
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
                                                             (int)C.Nc, xi_dev, nk);
     else
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, C.cpidx.p, C.LinvT.p,
+      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p,
                                                               C.Qinv.p, (int)C.Nc, xi_dev, nk);
   }
 
@@ -630,7 +654,9 @@ struct Engine : EngineBase {
     Level& Lv = *C.L[l];
     T* dst = which == 0 ? X(l) : which == 1 ? f[l].p : ext.p;
     launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_pack<T>, ref_dev, dst, Lv.g, C.K, per_chain);
+    if (l == 0 && which == 1) f0_shared = per_chain == 0;
   }
+  bool f_shared() const override { return f0_shared; }
   void unpack(int l, int which, double* ref_dev, double scale) override {
     Level& Lv = *C.L[l];
     const T* src = which == 0 ? X(l) : which == 1 ? f[l].p : r[l].p;
@@ -684,21 +710,22 @@ struct Engine : EngineBase {
       else launch(gr, 256, 0, C.stream, k_residual<double, 2, false>, x64, b64, r64, F.g, F.st);
     }
     if (C.m > 0) {
-      dim3 grid(C.m, C.K);
+      dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
       NoiseKey nk = key(0);
-      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.z.p);
+      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p);
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, C.sigma_inv_mat.p, C.sqrt_sigma.p, -1.0, nullptr,
              C.u.p, C.m, nk);
-      dim3 g2(nblocks(F.Brows.nrows), C.K);
+      dim3 g2(nblocks(F.Brows.nrows), (C.K + LR_CPU - 1) / LR_CPU);
       if (F.Brows.nrows)
         launch(g2, 256, 0, C.stream, k_lr_update<double>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
-               F.Brows.val.p, F.Brows.nrows, C.u.p, C.m);
+               F.Brows.val.p, F.Brows.nrows, C.u.p, C.m, C.K);
     }
   }
   void apply_neg_operator_fp64(const double* x64, double* y64) override { fine_residual_fp64(x64, nullptr, y64); }
   void set_f_from_fp64(const double* pad64) override {
     long long n = (long long)C.L[0]->g.cs * C.K;
     launch(nblocks(n), 256, 0, C.stream, k_convert<T>, f[0].p, pad64, n);
+    f0_shared = false;
   }
   void add_x_to_fp64(double* pad64) override {
     long long n = (long long)C.L[0]->g.cs * C.K;
@@ -721,6 +748,7 @@ struct mgmc_handle {
   hipGraph_t graph = nullptr;
   double* graph_obs = nullptr;
   int graph_valid = 0;
+  bool graph_fshared = false;  // level-0 f layout the graph was captured with
 
   ~mgmc_handle() {
     for (hipEvent_t e : C.ev) (void)hipEventDestroy(e);
@@ -792,7 +820,8 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
   for (int l = 0; l < C.nlevel; ++l) {
     const HostLevel& HL = C.H.levels[l];
     auto Lv = std::make_unique<Level>();
-    Lv->g = DevGeom{HL.g.dim, HL.g.n[0], HL.g.n[1], HL.g.n[2], HL.g.px, HL.g.py, HL.g.pz, (long long)HL.g.cstride};
+    Lv->g = DevGeom{HL.g.dim, HL.g.n[0], HL.g.n[1], HL.g.n[2], HL.g.px,
+                    HL.g.py,  HL.g.pz,   (long long)HL.g.cstride, (long long)HL.g.cstride};
     for (int i = 0; i < 27; ++i) Lv->st.c[i] = HL.st[i];
     Lv->st.diag = HL.diag;
     Lv->st.omega = par->omega;
@@ -959,7 +988,7 @@ static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph
     for (int s = 0; s < nsamples; ++s) record_cycle(h, obs_dev);
     return;
   }
-  if (!h->graph_valid || h->graph_obs != obs_dev) {
+  if (!h->graph_valid || h->graph_obs != obs_dev || h->graph_fshared != h->E->f_shared()) {
     if (h->gexec) HIP_CHECK(hipGraphExecDestroy(h->gexec));
     if (h->graph) HIP_CHECK(hipGraphDestroy(h->graph));
     h->gexec = nullptr;
@@ -969,6 +998,7 @@ static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph
     HIP_CHECK(hipStreamEndCapture(h->C.stream, &h->graph));
     HIP_CHECK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
     h->graph_obs = obs_dev;
+    h->graph_fshared = h->E->f_shared();
     h->graph_valid = 1;
   }
   for (int s = 0; s < nsamples; ++s) HIP_CHECK(hipGraphLaunch(h->gexec, h->C.stream));
