@@ -105,6 +105,14 @@ int mgmc_set_state(mgmc_handle* h, const double* x);
 int mgmc_get_state(mgmc_handle* h, double* x);
 /* Observables: values of x at `nprobe` fine vertices (reference order). */
 int mgmc_set_probes(mgmc_handle* h, int nprobe, const int64_t* vertices);
+/* Observables as sparse vectors: z_p = sum_e weights[e] x[vertices[e]] for
+ * e in [rowptr[p], rowptr[p+1]) (reference order; weights NULL = all 1).
+ * Replaces the sample_vector.dot(x) of measure_sampling_time /
+ * measure_convergence / measure_mean_squared_error
+ * (src/measurements/sampling_time.cc:40, convergence.cc:57,
+ * mean_squared_error.cc:45); a probe is one entry of weight 1. */
+int mgmc_set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const int64_t* vertices,
+                         const double* weights);
 /* Runs nsamples cycles on the device; if obs != NULL writes
  * obs[s][chain][probe] (host) after each cycle. use_graph = 1 replays a
  * captured hipGraph of the cycle. */

@@ -9,3 +9,4 @@ from .operators import (Lattice, Lattice2d, Lattice3d, MeasuredOperator, Shifted
                         measurement_vector, seeded_ball_measurements)
 from .sampler import LoopSolver, MultigridMCSampler, MultigridParameters, MultigridPreconditioner  # noqa: F401
 from .stats import tau_int_chains, tau_int_sokal, tau_int_window  # noqa: F401
+from . import measurements  # noqa: F401,E402

@@ -67,6 +67,7 @@ _SIGS = {
     "mgmc_set_state": [_vp, _vp],
     "mgmc_get_state": [_vp, _vp],
     "mgmc_set_probes": [_vp, C.c_int, _vp],
+    "mgmc_set_observables": [_vp, C.c_int, _vp, _vp, _vp],
     "mgmc_run": [_vp, C.c_int, _vp, C.c_int],
     "mgmc_run_dev": [_vp, C.c_int, _vp, C.c_int],
     "mgmc_set_counter": [_vp, C.c_uint64],

@@ -1011,14 +1011,21 @@ __global__ void __launch_bounds__(1024) k_residual_restrict_star3(const T* __res
 }
 
 // obs[row][c][p] = x_c[probe_p], row taken from the device run counter
+// obs[row][c][p] = sum_e w_e x_c[idx_e] over the entries of observable p
+// (a probe vertex is one entry of weight 1; a ball average is a measurement
+// vector), row taken from the device run counter. One wave per (p, c).
 template <typename T>
-__global__ void k_probe(const T* __restrict__ x, long long cs, const long long* __restrict__ pidx, int P, int K,
-                        double* __restrict__ obs, const uint32_t* __restrict__ ctr) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P * K) return;
-  int c = t / P, p = t % P;
-  uint32_t row = ctr[1];
-  obs[((long long)row * K + c) * P + p] = (double)x[c * cs + pidx[p]];
+__global__ void __launch_bounds__(64) k_observe(const T* __restrict__ x, long long cs,
+                                                const int32_t* __restrict__ optr,
+                                                const long long* __restrict__ oidx, const double* __restrict__ ow,
+                                                int P, int K, double* __restrict__ obs,
+                                                const uint32_t* __restrict__ ctr) {
+  const int p = blockIdx.x, c = blockIdx.y;
+  const T* xc = x + (long long)c * cs;
+  double s = 0.0;
+  for (int e = optr[p] + (int)threadIdx.x; e < optr[p + 1]; e += 64) s += ow[e] * (double)xc[oidx[e]];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) obs[((long long)ctr[1] * K + c) * P + p] = s;
 }
 
 // advance the sample counter and the observable row after a cycle

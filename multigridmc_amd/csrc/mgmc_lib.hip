@@ -116,7 +116,10 @@ struct Common {
   DevBuf<double> LinvT, Qinv;
   int64_t Nc = 0;
   DevBuf<uint32_t> ctr;  // [0] sample counter, [1] observable row
-  DevBuf<long long> probe_pidx;
+  // observables: sparse vectors over the fine vertices (CSR, padded indices)
+  DevBuf<int32_t> obs_ptr;
+  DevBuf<long long> obs_idx;
+  DevBuf<double> obs_w;
   int nprobe = 0;
   uint32_t tag = 0;
   // tuning: fused red-black sweep on 3D star levels, its threads per block
@@ -668,8 +671,8 @@ struct Engine : EngineBase {
 
   void probe(double* obs_dev) override {
     if (C.nprobe == 0 || !obs_dev) return;
-    launch(nblocks((long long)C.nprobe * C.K), 256, 0, C.stream, k_probe<T>, X(0), C.L[0]->g.cs, C.probe_pidx.p,
-                                                                          C.nprobe, C.K, obs_dev, C.ctr.p);
+    launch(dim3(C.nprobe, C.K), 64, 0, C.stream, k_observe<T>, X(0), C.L[0]->g.cs, C.obs_ptr.p, C.obs_idx.p,
+           C.obs_w.p, C.nprobe, C.K, obs_dev, C.ctr.p);
   }
 
   void ensure_ext(int l) {
@@ -959,18 +962,45 @@ int mgmc_sample(mgmc_handle* h, const double* f, int f_per_chain, double* x) {
   API_END
 }
 
+static void set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const int64_t* vertices,
+                            const double* weights) {
+  if (nobs < 0) throw Error(MGMC_ERR_ARG, "number of observables < 0");
+  const HostLevel& HL = h->C.H.levels[0];
+  if (nobs > 0 && (rowptr[0] != 0)) throw Error(MGMC_ERR_ARG, "observable rowptr[0] != 0");
+  const int64_t nnz = nobs ? rowptr[nobs] : 0;
+  if (nnz > INT32_MAX) throw Error(MGMC_ERR_ARG, "too many observable entries");
+  std::vector<int32_t> ptr(nobs + 1, 0);
+  std::vector<long long> idx(nnz);
+  std::vector<double> w(nnz);
+  for (int p = 0; p < nobs; ++p) {
+    if (rowptr[p + 1] < rowptr[p]) throw Error(MGMC_ERR_ARG, "observable rowptr not increasing");
+    ptr[p + 1] = (int32_t)rowptr[p + 1];
+  }
+  for (int64_t e = 0; e < nnz; ++e) {
+    if (vertices[e] < 0 || vertices[e] >= HL.g.N) throw Error(MGMC_ERR_ARG, "observable vertex out of range");
+    idx[e] = HL.g.padded(vertices[e]);
+    w[e] = weights ? weights[e] : 1.0;
+  }
+  h->C.obs_ptr.upload(ptr.data(), ptr.size());
+  h->C.obs_idx.upload(idx.data(), idx.size());
+  h->C.obs_w.upload(w.data(), w.size());
+  h->C.nprobe = nobs;
+  h->graph_valid = 0;
+}
+
 int mgmc_set_probes(mgmc_handle* h, int nprobe, const int64_t* vertices) {
   API_BEGIN
   if (nprobe < 0) throw Error(MGMC_ERR_ARG, "nprobe < 0");
-  const HostLevel& HL = h->C.H.levels[0];
-  std::vector<long long> p(nprobe);
-  for (int i = 0; i < nprobe; ++i) {
-    if (vertices[i] < 0 || vertices[i] >= HL.g.N) throw Error(MGMC_ERR_ARG, "probe vertex out of range");
-    p[i] = HL.g.padded(vertices[i]);
-  }
-  h->C.probe_pidx.upload(p.data(), p.size());
-  h->C.nprobe = nprobe;
-  h->graph_valid = 0;
+  std::vector<int64_t> rp(nprobe + 1);
+  for (int i = 0; i <= nprobe; ++i) rp[i] = i;
+  set_observables(h, nprobe, rp.data(), vertices, nullptr);
+  API_END
+}
+
+int mgmc_set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const int64_t* vertices,
+                         const double* weights) {
+  API_BEGIN
+  set_observables(h, nobs, rowptr, vertices, weights);
   API_END
 }
 

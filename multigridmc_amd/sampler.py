@@ -99,6 +99,14 @@ class _Handle:
     def set_tuning(self, key, value):
         check(_lib.lib().mgmc_set_tuning(self.h, key.encode(), int(value)))
 
+    def apply_operator(self, x):
+        """y = A x + B Sigma^{-1} B^T x on the GPU for every row of x
+        (LinearOperator::apply, src/linear_operator/linear_operator.hh:78-88)."""
+        xx = self._rows(x)
+        out = np.zeros_like(xx)
+        check(_lib.lib().mgmc_apply_operator(self.h, ptr(xx), ptr(out)))
+        return out.reshape(np.shape(x)) if np.size(x) == out.size else out
+
     def ndof(self, level):
         n = C.c_int64()
         check(_lib.lib().mgmc_ndof(self.h, level, C.byref(n)))
@@ -211,11 +219,21 @@ class MultigridMCSampler(_Handle):
         check(_lib.lib().mgmc_level_stencil(self.h, level, ptr(st), C.byref(nc)))
         return st, nc.value
 
-    def apply_operator(self, x):
-        xx = self._rows(x)
-        out = np.zeros_like(xx)
-        check(_lib.lib().mgmc_apply_operator(self.h, ptr(xx), ptr(out)))
-        return out
+    def set_observables(self, observables):
+        """Observables z_p = b_p . x, each given as (vertex indices, weights)
+        (e.g. measurement_vector(lattice, x0, radius)); run() then records
+        obs[sample, chain, p]."""
+        rp = [0]
+        idx, w = [], []
+        for v, wt in observables:
+            idx.append(i64(v))
+            w.append(f64(wt))
+            rp.append(rp[-1] + len(v))
+        rp = i64(rp)
+        vi = i64(np.concatenate(idx)) if idx else i64([])
+        wv = f64(np.concatenate(w)) if w else f64([])
+        check(_lib.lib().mgmc_set_observables(self.h, len(observables), ptr(rp), ptr(vi), ptr(wv)))
+        self.nprobe = len(observables)
 
 
 class MultigridPreconditioner(_Handle):
