@@ -584,6 +584,10 @@ struct Engine : EngineBase {
   }
 
   void coarse(int mode, const double* xi_dev) override {
+    if (C.Nc == 0)
+      throw Error(MGMC_ERR_UNSUPPORTED,
+                  "no dense coarse factor: the coarsest level has more than 2048 unknowns (the multigrid "
+                  "preconditioner always solves it with Cholesky); add levels");
     int l = C.nlevel - 1;
     Level& Lv = *C.L[l];
     size_t sh = 2 * (size_t)C.Nc * sizeof(double);
@@ -814,7 +818,7 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
   C.k1 = (uint32_t)(par->seed >> 32);
   HostProblem hp{prob->dim, {prob->n[0], prob->n[1], prob->n[2]}, prob->kappa_sq, prob->A_rowptr, prob->A_col,
                  prob->A_val, prob->m, prob->B_rowptr, prob->B_col, prob->B_val, prob->sigma};
-  C.H = build_hierarchy(hp, par->nlevel, par->omega);
+  C.H = build_hierarchy(hp, par->nlevel, par->omega, par->coarse_solver == MGMC_COARSE_CHOLESKY);
   if (par->stream) C.stream = (hipStream_t)par->stream;
   else {
     HIP_CHECK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
@@ -858,7 +862,8 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     C.u.alloc((size_t)C.m * C.K);
   }
   C.Nc = C.H.coarse.Nc;
-  if (C.Nc > 2048) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level has more than 2048 unknowns; add levels");
+  if (C.Nc > kMaxCoarseKernel)
+    throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level has more than 2048 unknowns; add levels");
   {
     std::vector<long long> pi(C.H.coarse.pidx.begin(), C.H.coarse.pidx.end());
     C.cpidx.upload(pi.data(), pi.size());

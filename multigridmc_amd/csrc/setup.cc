@@ -396,7 +396,7 @@ static void build_coarse(const HostLevel& L, const std::vector<Column>& cols, co
                          HostCoarse& C) {
   const Geom& g = L.g;
   int64_t Nc = g.N;
-  if (Nc > 8192) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level too large for the dense Cholesky sampler");
+  if (Nc > kMaxCoarse) throw Error(MGMC_ERR_UNSUPPORTED, "coarsest level too large for the dense Cholesky sampler");
   C.Nc = Nc;
   C.pidx.resize(Nc);
   std::unordered_map<int64_t, int64_t> ref;
@@ -456,7 +456,7 @@ static void build_coarse(const HostLevel& L, const std::vector<Column>& cols, co
     }
 }
 
-HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega) {
+HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bool coarse_cholesky) {
   if (nlevel < 1) throw Error(MGMC_ERR_ARG, "nlevel must be >= 1");
   if (!(omega > 0.0 && omega < 2.0)) throw Error(MGMC_ERR_ARG, "omega must lie in (0, 2)");
   if (p.m < 0) throw Error(MGMC_ERR_ARG, "m must be >= 0");
@@ -501,7 +501,11 @@ HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega) {
       set_columns(L, cols);
       compute_lowrank(L, cols, H.sigma, omega);
     }
-    if (l == nlevel - 1) build_coarse(L, cols, H.sigma, H.coarse);
+    // The dense coarse factor serves the Cholesky sampler and the
+    // preconditioner's coarse solve (always Cholesky, as in
+    // multigrid_preconditioner.cc:41-45). With an SSOR coarse sampler it is
+    // built only when the coarsest level is small enough.
+    if (l == nlevel - 1 && (coarse_cholesky || L.g.N <= kMaxCoarseKernel)) build_coarse(L, cols, H.sigma, H.coarse);
   }
   return H;
 }

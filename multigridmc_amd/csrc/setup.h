@@ -92,7 +92,13 @@ struct HostHierarchy {
   double omega = 1.0;
 };
 
-HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega);
+// Largest coarsest level (unknowns) for the dense Cholesky factor; the
+// coarse kernel stages 2 Nc doubles in LDS, which bounds it by kMaxCoarseKernel.
+constexpr int64_t kMaxCoarse = 8192;
+constexpr int64_t kMaxCoarseKernel = 2048;
+// coarse_cholesky: the coarsest level is sampled with the Cholesky sampler
+// (the factor is then required); otherwise it is built when it fits.
+HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bool coarse_cholesky = true);
 
 // Galerkin coarse stencil of a constant fine stencil with linear interpolation.
 void galerkin_stencil(int dim, const double* fine, double* coarse);

@@ -4,6 +4,7 @@ Only setup lives here (NumPy/SciPy, FP64): lattices, the shifted-Laplace FD
 prior and the ball-average measurement matrix B. Everything that runs per
 sample is in the HIP library.
 """
+import itertools
 import math
 
 import numpy as np
@@ -112,6 +113,86 @@ class ShiftedLaplaceFDOperator:
                 cols.append(L.vertex_index(j[ok]))
                 vals.append(np.full(ok.sum(), off[d]))
         return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+
+class ShiftedLaplaceFEMOperator:
+    """Prior precision of multilinear (Q1) finite elements for -Delta + kappa^2
+    with Dirichlet elimination (src/linear_operator/shiftedlaplace_fem_operator.cc:9-141),
+    constant kappa^2 = 1 / Lambda^2. Assembled as the reference does, cell by
+    cell with 2^d-point Gauss-Legendre quadrature (exact for Q1); with constant
+    coefficients every interior row holds the same 3^d stencil, which is what
+    the HIP library reads back from the CSR matrix (a box-stencil fine level)."""
+
+    m_lowrank = 0
+
+    def __init__(self, lattice, kappa_sq):
+        self.lattice = lattice
+        self.kappa_sq = float(kappa_sq)
+
+    def get_ndof(self):
+        return self.lattice.Nvertex
+
+    def local_matrix(self):
+        """Element matrix M[alpha, beta] (alpha, beta in {0,1}^d, first
+        component slowest) including the cell volume."""
+        d = self.lattice.dim
+        h = np.array([1.0 / v for v in self.lattice.n])
+        hinv2 = 1.0 / (h * h)
+        cv = float(np.prod(h))
+        bits = np.array(list(itertools.product((0, 1), repeat=d)))
+        g1 = np.array([0.5 * (1.0 - 1.0 / math.sqrt(3.0)), 0.5 * (1.0 + 1.0 / math.sqrt(3.0))])
+        qp = g1[bits]                      # (nq, d) quadrature points
+        qw = np.full(len(qp), 0.5 ** d)    # weights on [0, 1]^d
+        # phi_alpha(x) = prod_d (x_d if alpha_d else 1 - x_d)
+        fac = np.where(bits[:, None, :] == 1, qp[None, :, :], 1.0 - qp[None, :, :])  # (nb, nq, d)
+        phi = np.prod(fac, axis=2)
+        dfac = np.where(bits[:, None, :] == 1, 1.0, -1.0) * np.ones_like(fac)
+        grad = np.empty_like(fac)
+        for j in range(d):
+            others = np.prod(np.delete(fac, j, axis=2), axis=2) if d > 1 else np.ones(fac.shape[:2])
+            grad[:, :, j] = dfac[:, :, j] * others
+        M = np.zeros((len(bits), len(bits)))
+        for a in range(len(bits)):
+            for b in range(len(bits)):
+                val = 0.0
+                for q in range(len(qp)):
+                    val += (self.kappa_sq * phi[a, q] * phi[b, q] + np.dot(grad[a, q], hinv2 * grad[b, q])) * qw[q]
+                M[a, b] = val * cv
+        return M, bits
+
+    def stencil(self):
+        """3^d stencil (offsets in {-1,0,1}^d, first component slowest) of an
+        interior row: the sum of the element entries of all cells holding
+        both vertices."""
+        M, bits = self.local_matrix()
+        d = self.lattice.dim
+        offs = list(itertools.product((-1, 0, 1), repeat=d))
+        st = {}
+        for s in offs:
+            v = 0.0
+            for a, alpha in enumerate(bits):
+                beta = alpha + np.array(s)
+                if np.all((beta >= 0) & (beta <= 1)):
+                    b = int(np.dot(beta, 1 << np.arange(d)[::-1]))
+                    v += M[a, b]
+            st[s] = v
+        return st
+
+    def get_sparse(self):
+        L = self.lattice
+        N = L.Nvertex
+        idx = L.vertex_euclidean(np.arange(N))
+        rows, cols, vals = [], [], []
+        # offsets are given with the first lattice component slowest
+        for s, v in self.stencil().items():
+            j = idx + np.array(s)
+            ok = np.all((j >= 1) & (j <= np.array(L.n) - 1), axis=1)
+            rows.append(np.arange(N)[ok])
+            cols.append(L.vertex_index(j[ok]))
+            vals.append(np.full(ok.sum(), v))
+        A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+        A.sort_indices()
+        return A
 
 
 def V_sphere(radius, dim):

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, f64, i64, ptr
+from .operators import ShiftedLaplaceFDOperator
 
 
 @dataclass
@@ -32,7 +33,10 @@ class MultigridParameters:
 
 
 def _problem(op):
-    """mgmc_problem for a ShiftedLaplaceFDOperator or MeasuredOperator."""
+    """mgmc_problem for a prior operator (ShiftedLaplaceFDOperator: assembled
+    by the library from kappa^2; any other constant-coefficient operator, e.g.
+    ShiftedLaplaceFEMOperator: passed as its CSR matrix) or a MeasuredOperator
+    wrapping one."""
     lat = op.lattice
     p = _lib.Problem()
     p.dim = lat.dim
@@ -40,6 +44,13 @@ def _problem(op):
         p.n[d] = lat.n[d] if d < lat.dim else 1
     p.kappa_sq = op.kappa_sq
     keep = []
+    prior = getattr(op, "base", op)
+    if not isinstance(prior, ShiftedLaplaceFDOperator):
+        A = prior.get_sparse().tocsr()
+        A.sort_indices()
+        rp, ci, v = i64(A.indptr), i64(A.indices), f64(A.data)
+        keep += [rp, ci, v]
+        p.A_rowptr, p.A_col, p.A_val = ptr(rp), ptr(ci), ptr(v)
     m = getattr(op, "m_lowrank", 0)
     p.m = m
     if m:
@@ -234,6 +245,19 @@ class MultigridMCSampler(_Handle):
         wv = f64(np.concatenate(w)) if w else f64([])
         check(_lib.lib().mgmc_set_observables(self.h, len(observables), ptr(rp), ptr(vi), ptr(wv)))
         self.nprobe = len(observables)
+
+
+class SSORSampler(MultigridMCSampler):
+    """Standalone SSOR Gibbs sampler (src/sampler/ssor_sampler.cc:9-15): each
+    apply is `nsmooth` x (forward SOR sweep, backward SOR sweep) with noise
+    and the low-rank correction, on the fine lattice only. Runs the same
+    kernels as the multigrid cycle (a one-level hierarchy whose level is
+    sampled by SSOR sweeps)."""
+
+    def __init__(self, linear_operator, omega=1.0, nsmooth=1, nchains=1, precision="fp64", seed=0x5eed, stream=None):
+        params = MultigridParameters(nlevel=1, smoother="SSOR", npresmooth=0, npostsmooth=0, ncoarsesmooth=nsmooth,
+                                     omega=omega, coarse_solver="SSOR")
+        super().__init__(linear_operator, params, nchains, precision, seed, stream)
 
 
 class MultigridPreconditioner(_Handle):
