@@ -119,6 +119,12 @@ int mgmc_set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const 
 int mgmc_run(mgmc_handle* h, int nsamples, double* obs, int use_graph);
 /* Same, with the observable buffer in device memory (nsamples x nchains x nprobe). */
 int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph);
+/* Pointwise posterior statistics (posterior_statistics,
+ * src/measurements/posterior_statistics.cc:4-55): runs `nsamples` cycles from
+ * the current state and returns the mean and variance of every fine vertex
+ * over all samples and chains (reference order, N values each; either
+ * pointer may be NULL). */
+int mgmc_moments(mgmc_handle* h, int nsamples, double* mean, double* variance);
 /* Reset the sample counter that, with the seed, keys the Philox noise. */
 int mgmc_set_counter(mgmc_handle* h, uint64_t sample_counter);
 
@@ -160,16 +166,37 @@ int mgmc_coarse(mgmc_handle* h, const double* f, double* x, int xi_mode, const d
  * fine vertex (reference order) the normal of sweep `tag` at the current
  * sample counter. out: nchains x N. */
 int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out);
+/* ---- host-side operator setup for C/C++ callers (no device work) ---- */
+
+/* Ball-average measurement vector of MeasuredOperator::measurement_vector
+ * (src/linear_operator/measured_operator.cc:69-170): top-hat average over the
+ * ball (x0, radius) tested against the multilinear hat functions, 2^d-point
+ * Gauss quadrature per cell; radius < 1e-12 gives the nearest vertex. Writes
+ * ascending reference vertex indices and values; *nnz receives the length
+ * (MGMC_ERR_ARG if it exceeds `capacity`). n holds dim cell counts. */
+int mgmc_measurement_vector(int dim, const int* n, const double* x0, double radius, int64_t capacity, int64_t* idx,
+                            double* val, int64_t* nnz);
+/* Constant 3^d stencil of the Q1 finite-element prior -Delta + kappa^2
+ * (src/linear_operator/shiftedlaplace_fem_operator.cc:9-141), index
+ * (dz+1)*9 + (dy+1)*3 + (dx+1) (3D) or (dy+1)*3 + (dx+1) (2D); building A as a
+ * CSR matrix from it and passing it to mgmc_create gives the FEM prior. */
+int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil);
+
 /* Export the level's stencil as used by the kernels (27 or 9 entries, x fastest,
  * offsets -1..1 per direction) and the colour count (2 = red-black). */
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);
 
-/* Launch-shape tuning (no reference counterpart): key "fused_sweep" (0/1:
- * sweep 3D star levels with the single-pass red-black kernel), "plane_sweep"
- * (0/1: sweep 3D box levels in two plane-parity launches), "rb_threads"
+/* Launch-shape tuning (no reference counterpart). Keys: "fused_sweep" (0/1:
+ * sweep 3D star levels with the single-pass red-black kernel), "rb_threads"
  * (threads per workgroup of that kernel, 64..1024), "rb_blocks" (workgroups
- * it aims for; more z-segments when fewer tiles x chains exist). Results do
- * not depend on these settings beyond rounding. */
+ * it aims for; more z-segments when fewer tiles x chains exist),
+ * "fold_prolongation" (0/1: add the coarse correction inside the first
+ * post-smoothing sweep), "fused_residual" (0/1: residual + restriction in one
+ * pass on 3D star levels), "rr_threads" (its threads per workgroup,
+ * 64..1024), "plane_sweep" (0/1: sweep 3D box levels in two plane-parity
+ * launches), "plane_sweep_min" (fewest plane x chain workgroups for that),
+ * "plane_sweep_stage3" (0/1: planes k-1, k, k+1 in LDS). Results do not
+ * depend on these settings beyond rounding. */
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
 /* Profiling aid: with enable = 1 every level-0 sweep launch (one fused

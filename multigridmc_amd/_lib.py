@@ -70,6 +70,7 @@ _SIGS = {
     "mgmc_set_observables": [_vp, C.c_int, _vp, _vp, _vp],
     "mgmc_run": [_vp, C.c_int, _vp, C.c_int],
     "mgmc_run_dev": [_vp, C.c_int, _vp, C.c_int],
+    "mgmc_moments": [_vp, C.c_int, _vp, _vp],
     "mgmc_set_counter": [_vp, C.c_uint64],
     "mg_precondition": [_vp, _vp, _vp],
     "mg_solve": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],
@@ -80,6 +81,8 @@ _SIGS = {
     "mgmc_coarse": [_vp, _vp, _vp, C.c_int, _vp],
     "mgmc_normals": [_vp, C.c_uint32, _vp],
     "mgmc_level_stencil": [_vp, C.c_int, _vp, C.POINTER(C.c_int)],
+    "mgmc_measurement_vector": [C.c_int, _vp, _vp, C.c_double, C.c_int64, _vp, _vp, C.POINTER(C.c_int64)],
+    "mgmc_fem_stencil": [C.c_int, _vp, C.c_double, _vp],
     "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],
     "mgmc_profile": [_vp, C.c_int],
     "mgmc_profile_result": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)],

@@ -15,18 +15,37 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-W
 
 def build(verbose=True):
     obj = os.path.join(CSRC, "setup.o")
+    oobj = os.path.join(CSRC, "operators.o")
     hobj = os.path.join(CSRC, "mgmc_lib.o")
     cmds = [
         ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "setup.cc"), "-o", obj],
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "operators.cc"), "-o", oobj],
         [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mgmc_lib.hip"), "-o", hobj],
-        [HIPCC, "--offload-arch=gfx950", "-shared", hobj, obj, "-o", LIB + ".tmp"],
+        [HIPCC, "--offload-arch=gfx950", "-shared", hobj, obj, oobj, "-o", LIB + ".tmp"],
     ]
     for c in cmds:
         if verbose:
             print(" ".join(c), flush=True)
         subprocess.check_call(c)
     os.replace(LIB + ".tmp", LIB)
+    build_drivers(verbose)
     return LIB
+
+
+def build_drivers(verbose=True):
+    """driver_mgmc and driver_mg (host C++ over the C ABI) into multigridmc_amd/bin."""
+    host = os.path.join(CSRC, "host")
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    common = [os.path.join(host, "config.cc"), os.path.join(host, "driver_common.cc")]
+    for name in ("driver_mgmc", "driver_mg"):
+        out = os.path.join(bindir, name)
+        c = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(host, name + ".cc")] + common + [
+            "-L" + HERE, "-lmgmc_amd", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(c), flush=True)
+        subprocess.check_call(c)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -1028,6 +1028,23 @@ __global__ void __launch_bounds__(64) k_observe(const T* __restrict__ x, long lo
   if (threadIdx.x == 0) obs[((long long)ctr[1] * K + c) * P + p] = s;
 }
 
+// m1[i] += sum_c x_c[i], m2[i] += sum_c x_c[i]^2 over the padded level-0 box
+// (pointwise moments for the posterior mean and variance fields)
+template <typename T>
+__global__ void __launch_bounds__(256) k_moments(const T* __restrict__ x, long long cs, int K, long long vol,
+                                                 double* __restrict__ m1, double* __restrict__ m2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vol) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int c = 0; c < K; ++c) {
+    const double v = (double)x[c * cs + i];
+    s1 += v;
+    s2 += v * v;
+  }
+  m1[i] += s1;
+  m2[i] += s2;
+}
+
 // advance the sample counter and the observable row after a cycle
 __global__ void k_advance(uint32_t* ctr) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -162,6 +162,8 @@ struct EngineBase {
   virtual void add_x_to_fp64(double* pad64) = 0;
   virtual void retune() = 0;
   virtual bool f_shared() const = 0;
+  virtual void accumulate_moments(double* m1, double* m2) = 0;
+  virtual void unpack_moment(const double* m, double* ref_dev, double scale) = 0;
 };
 
 // A level field for all chains with a zeroed guard of GUARD elements on both
@@ -664,6 +666,15 @@ struct Engine : EngineBase {
     if (l == 0 && which == 1) f0_shared = per_chain == 0;
   }
   bool f_shared() const override { return f0_shared; }
+  void accumulate_moments(double* m1, double* m2) override {
+    const DevGeom& g = C.L[0]->g;
+    long long vol = (long long)g.px * g.py * g.pz;
+    launch(nblocks(vol), 256, 0, C.stream, k_moments<T>, X(0), g.cs, C.K, vol, m1, m2);
+  }
+  void unpack_moment(const double* m, double* ref_dev, double scale) override {
+    Level& Lv = *C.L[0];
+    launch(nblocks(Lv.N), 256, 0, C.stream, k_unpack<double>, m, ref_dev, Lv.g, 1, scale);
+  }
   void unpack(int l, int which, double* ref_dev, double scale) override {
     Level& Lv = *C.L[l];
     const T* src = which == 0 ? X(l) : which == 1 ? f[l].p : r[l].p;
@@ -903,6 +914,29 @@ int mgmc_ndof(const mgmc_handle* h, int level, int64_t* n) {
   API_END
 }
 
+int mgmc_measurement_vector(int dim, const int* n, const double* x0, double radius, int64_t capacity, int64_t* idx,
+                            double* val, int64_t* nnz) {
+  API_BEGIN
+  if (!n || !x0 || !nnz) throw Error(MGMC_ERR_ARG, "null argument");
+  for (int d = 0; d < dim && d < 3; ++d)
+    if (n[d] < 2) throw Error(MGMC_ERR_ARG, "lattice needs at least 2 cells per direction");
+  std::vector<int64_t> vi;
+  std::vector<double> vv;
+  measurement_vector(dim, n, x0, radius, vi, vv);
+  *nnz = (int64_t)vi.size();
+  if ((int64_t)vi.size() > capacity) throw Error(MGMC_ERR_ARG, "capacity too small for the measurement vector");
+  std::copy(vi.begin(), vi.end(), idx);
+  std::copy(vv.begin(), vv.end(), val);
+  API_END
+}
+
+int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil) {
+  API_BEGIN
+  if (!n || !stencil) throw Error(MGMC_ERR_ARG, "null argument");
+  fem_stencil(dim, n, kappa_sq, stencil);
+  API_END
+}
+
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours) {
   API_BEGIN
   if (!h || level < 0 || level >= h->C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
@@ -1037,6 +1071,38 @@ static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph
     h->graph_valid = 1;
   }
   for (int s = 0; s < nsamples; ++s) HIP_CHECK(hipGraphLaunch(h->gexec, h->C.stream));
+}
+
+int mgmc_moments(mgmc_handle* h, int nsamples, double* mean, double* variance) {
+  API_BEGIN
+  if (nsamples < 1) throw Error(MGMC_ERR_ARG, "nsamples < 1");
+  const DevGeom& g = h->C.L[0]->g;
+  const size_t vol = (size_t)g.px * g.py * g.pz;
+  const int64_t N = h->C.L[0]->N;
+  DevBuf<double> m1, m2, out;
+  m1.alloc(vol);
+  m2.alloc(vol);
+  out.alloc((size_t)N);
+  HIP_CHECK(hipMemsetAsync(m1.p, 0, vol * sizeof(double), h->C.stream));
+  HIP_CHECK(hipMemsetAsync(m2.p, 0, vol * sizeof(double), h->C.stream));
+  for (int s = 0; s < nsamples; ++s) {
+    h->E->cycle_sample();
+    h->E->accumulate_moments(m1.p, m2.p);
+    launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+  }
+  const double scale = 1.0 / ((double)nsamples * h->C.K);
+  std::vector<double> e1((size_t)N), e2((size_t)N);
+  h->E->unpack_moment(m1.p, out.p, scale);
+  HIP_CHECK(hipMemcpyAsync(e1.data(), out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  h->E->unpack_moment(m2.p, out.p, scale);
+  HIP_CHECK(hipMemcpyAsync(e2.data(), out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
+  sync(h);
+  for (int64_t i = 0; i < N; ++i) {
+    if (mean) mean[i] = e1[i];
+    if (variance) variance[i] = e2[i] - e1[i] * e1[i];
+  }
+  API_END
 }
 
 int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph) {

@@ -92,6 +92,12 @@ struct HostHierarchy {
   double omega = 1.0;
 };
 
+// Ball-average measurement vector (reference vertex indices ascending, values)
+// and the Q1 FEM stencil; operators.cc.
+void measurement_vector(int dim, const int* n, const double* x0, double radius, std::vector<int64_t>& idx,
+                        std::vector<double>& val);
+void fem_stencil(int dim, const int* n, double kappa_sq, double* st);
+
 // Largest coarsest level (unknowns) for the dense Cholesky factor; the
 // coarse kernel stages 2 Nc doubles in LDS, which bounds it by kMaxCoarseKernel.
 constexpr int64_t kMaxCoarse = 8192;

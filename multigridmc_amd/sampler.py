@@ -177,6 +177,13 @@ class MultigridMCSampler(_Handle):
         check(_lib.lib().mgmc_run(self.h, nsamples, ptr(obs) if record and obs.size else None, int(use_graph)))
         return obs
 
+    def moments(self, nsamples):
+        """Pointwise mean and variance over `nsamples` further cycles of all
+        chains (posterior_statistics, src/measurements/posterior_statistics.cc)."""
+        mean, var = np.zeros(self.N), np.zeros(self.N)
+        check(_lib.lib().mgmc_moments(self.h, nsamples, ptr(mean), ptr(var)))
+        return mean, var
+
     def run_dev(self, nsamples, obs_dev_ptr=None, use_graph=True):
         check(_lib.lib().mgmc_run_dev(self.h, nsamples, obs_dev_ptr, int(use_graph)))
 
