@@ -120,11 +120,37 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
 // global memory). Outer margins of the first / last group are never used.
 template <int IPAR, typename T, int NOISE>
 __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* zm, const T* zp, const DevGeom& g,
-                                                  const StencilT<T>& st, const T* fb, const float* xi, const T* extg,
-                                                  int i0, int gx, int ngx) {
+                                                  const StencilT<T>& st, const T* __restrict__ coef, const T* fb,
+                                                  const float* xi, const T* extg, int i0, int gx, int ngx) {
   T acc[4];
 #pragma unroll
   for (int l = 0; l < 4; ++l) acc[l] = (T)0;
+  if constexpr (sizeof(T) == 8) {
+  // FP64: planes one at a time (keeps one plane's rows live: 62 instead of
+  // 128+ VGPRs, no spills); the centre row is read back from LDS, where it
+  // still holds the values of this phase
+#pragma unroll 1
+  for (int dz = -1; dz <= 1; ++dz) {
+    const T* pl = dz < 0 ? zm : dz > 0 ? zp : row;
+    const T* cz = coef + (dz + 1) * 9;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      T w[10];
+      const T* p = pl + dy * g.px;
+      load8<T>(p, w + 1);
+      w[0] = gx > 0 ? p[-1] : (T)0;
+      w[9] = gx < ngx - 1 ? p[8] : (T)0;
+      const T a0 = cz[(dy + 1) * 3 + 0];
+      const T a1 = cz[(dy + 1) * 3 + 1];
+      const T a2 = cz[(dy + 1) * 3 + 2];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int q = 2 * l + IPAR;
+        acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
+      }
+    }
+  }
+  } else {
 #pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
 #pragma unroll
@@ -148,6 +174,7 @@ __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* 
         acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
       }
     }
+  }
   }
   const T om_d = st.inv_diag_omega;
 #pragma unroll
@@ -826,7 +853,8 @@ __global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* 
 template <typename T, int NOISE, bool STAGE3>
 __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
                                                            const T* __restrict__ ext, DevGeom g, StencilT<T> st,
-                                                           int kpar, int forward, int nplanes, int hth, NoiseKey nk) {
+                                                           const T* __restrict__ coef, int kpar, int forward,
+                                                           int nplanes, int hth, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
   // STAGE3: planes k-1, k, k+1 in LDS (py x px each); else plane k only
   const size_t psz = (size_t)g.py * g.px;
@@ -873,9 +901,9 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     const T* zm = STAGE3 ? row - psz : x + base - (long long)g.px * g.py;
     const T* zp = STAGE3 ? row + psz : x + base + (long long)g.px * g.py;
     if (ipar == 0)
-      box_plane_update4<0, T, NOISE>(xc, row, zm, zp, g, st, fb, xi, eg, 8 * gx, gx, ngx);
+      box_plane_update4<0, T, NOISE>(xc, row, zm, zp, g, st, coef, fb, xi, eg, 8 * gx, gx, ngx);
     else
-      box_plane_update4<1, T, NOISE>(xc, row, zm, zp, g, st, fb, xi, eg, 8 * gx, gx, ngx);
+      box_plane_update4<1, T, NOISE>(xc, row, zm, zp, g, st, coef, fb, xi, eg, 8 * gx, gx, ngx);
     store8<T>(row, xc + 1);
   }
   __syncthreads();

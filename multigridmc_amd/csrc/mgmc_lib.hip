@@ -131,7 +131,7 @@ struct Common {
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
   int tune_rr_threads = 512;
   int tune_planes = 1;        // plane-parity sweeps on 3D box levels
-  int tune_planes_min = 256;  // ... when planes x chains reaches this many workgroups
+  int tune_planes_min = 1;    // ... when planes x chains reaches this many workgroups
   int tune_planes_stage3 = 1; // ... with planes k-1, k, k+1 staged in LDS when they fit
   // optional HIP-event timing of the level-0 colour-phase launches
   bool prof = false;
@@ -192,6 +192,7 @@ struct Engine : EngineBase {
   // calls the state is always in x (normalise()).
   std::vector<Field<T>> x2;
   std::vector<int> cur;
+  std::vector<DevBuf<T>> coef;  // the level's 27 stencil coefficients in precision T
   Field<T> ext;  // external noise staging (test hooks)
 
   T* X(int l) { return cur[l] ? x2[l].p : x[l].p; }
@@ -214,6 +215,12 @@ struct Engine : EngineBase {
     cur.assign(C.nlevel, 0);
     pending.assign(C.nlevel, 0);
     pending_alpha.assign(C.nlevel, 0.0);
+    coef.resize(C.nlevel);
+    for (int l = 0; l < C.nlevel; ++l) {
+      T c27[27];
+      for (int e = 0; e < 27; ++e) c27[e] = (T)C.L[l]->st.c[e];
+      coef[l].upload(c27, 27);
+    }
     for (int l = 0; l < C.nlevel; ++l) {
       size_t n = (size_t)C.L[l]->g.cs * C.K;
       x[l].alloc(n, C.stream);
@@ -330,8 +337,8 @@ struct Engine : EngineBase {
       if (nplanes <= 0) continue;
       dim3 grid((unsigned)(nplanes * C.K));
 #define MGMC_PLANES(NOISE, S3)                                                                                  \
-  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, geo(l), stt, kpar, \
-         (int)forward, nplanes, hth, nk)
+  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, geo(l), stt, coef[l].p, \
+         kpar, (int)forward, nplanes, hth, nk)
       if (stage3) {
         if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, true);
         else if (noise == NOISE_PHILOX) MGMC_PLANES(NOISE_PHILOX, true);
