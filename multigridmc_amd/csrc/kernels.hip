@@ -118,17 +118,17 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
 // plane is in LDS (`row` = the group in LDS, xc its values with margins);
 // zm / zp point at the same group in the planes below / above (LDS or
 // global memory). Outer margins of the first / last group are never used.
-template <int IPAR, typename T, int NOISE>
+template <int IPAR, typename T, int NOISE, bool ROLLED = (sizeof(T) == 8)>
 __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* zm, const T* zp, const DevGeom& g,
                                                   const StencilT<T>& st, const T* __restrict__ coef, const T* fb,
                                                   const float* xi, const T* extg, int i0, int gx, int ngx) {
   T acc[4];
 #pragma unroll
   for (int l = 0; l < 4; ++l) acc[l] = (T)0;
-  if constexpr (sizeof(T) == 8) {
-  // FP64: planes one at a time (keeps one plane's rows live: 62 instead of
-  // 128+ VGPRs, no spills); the centre row is read back from LDS, where it
-  // still holds the values of this phase
+  if constexpr (ROLLED) {
+  // planes one at a time (keeps one plane's rows live: 62 instead of 124
+  // VGPRs in FP32, no spills in FP64); the centre row is read back from LDS,
+  // where it still holds the values of this phase
 #pragma unroll 1
   for (int dz = -1; dz <= 1; ++dz) {
     const T* pl = dz < 0 ? zm : dz > 0 ? zp : row;
@@ -870,18 +870,39 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
   const int chain = blockIdx.x / nplanes;
   const int k = kpar ? 2 * pidx + 1 : 2 * pidx + 2;
   const long long base = (long long)chain * g.cs + ((long long)k * g.py + j) * g.px + 8 * gx;
-  // stage plane k (and k +- 1), boundary rows 0 and n1 included, in LDS
-  for (int dz = STAGE3 ? -1 : 0; dz <= (STAGE3 ? 1 : 0); ++dz)
-    for (int e = tid; e < g.py * ngx; e += blockDim.x) {
-      const int rr = e / ngx, gg = e % ngx;
-      T v[8];
-      load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, v);
-      store8<T>(P + dz * (long long)psz + (size_t)rr * g.px + 8 * gg, v);
-    }
-  T fb[8];
-  if (active) load8<T>(f + base + (long long)chain * (g.fcs - g.cs), fb);
   const uint32_t sample = *nk.sample;
   const int jpar = j & 1;
+  // stage plane k (and k +- 1), boundary rows 0 and n1 included, in LDS: the
+  // loads are issued first and the normals of the thread's two colour phases
+  // are drawn while they are in flight
+  const int ngrp = g.py * ngx;
+  const bool one = ngrp <= (int)blockDim.x;  // one group per thread and plane
+  T v[STAGE3 ? 3 : 1][8];
+  const int lr = tid / ngx, lg = tid % ngx;
+  if (one && tid < ngrp)
+    for (int d = 0; d < (STAGE3 ? 3 : 1); ++d)
+      load8<T>(x + (long long)chain * g.cs + ((long long)(k + d - (STAGE3 ? 1 : 0)) * g.py + lr) * g.px + 8 * lg, v[d]);
+  T fb[8];
+  if (active) load8<T>(f + base + (long long)chain * (g.fcs - g.cs), fb);
+  float xi[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+  if (NOISE == NOISE_PHILOX && active && 8 * gx <= g.n0 - 1)
+    for (int ipar = 0; ipar < 2; ++ipar) {
+      uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + gx) << 1) | (uint32_t)ipar;
+      normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xi[ipar]);
+    }
+  if (one) {
+    if (tid < ngrp)
+      for (int d = 0; d < (STAGE3 ? 3 : 1); ++d)
+        store8<T>(P + (d - (STAGE3 ? 1 : 0)) * (long long)psz + (size_t)lr * g.px + 8 * lg, v[d]);
+  } else {
+    for (int dz = STAGE3 ? -1 : 0; dz <= (STAGE3 ? 1 : 0); ++dz)
+      for (int e = tid; e < ngrp; e += blockDim.x) {
+        const int rr = e / ngx, gg = e % ngx;
+        T w[8];
+        load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, w);
+        store8<T>(P + dz * (long long)psz + (size_t)rr * g.px + 8 * gg, w);
+      }
+  }
   const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
   for (int ph = 0; ph < 4; ++ph) {
     const int q = forward ? ph : 3 - ph;  // in-plane colour: ipar = q & 1, jpar = q >> 1
@@ -893,17 +914,12 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     load8<T>(row, xc + 1);
     xc[0] = gx > 0 ? row[-1] : (T)0;
     xc[9] = gx < ngx - 1 ? row[8] : (T)0;
-    float xi[4] = {0.f, 0.f, 0.f, 0.f};
-    if (NOISE == NOISE_PHILOX) {
-      uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + gx) << 1) | (uint32_t)ipar;
-      normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xi);
-    }
     const T* zm = STAGE3 ? row - psz : x + base - (long long)g.px * g.py;
     const T* zp = STAGE3 ? row + psz : x + base + (long long)g.px * g.py;
     if (ipar == 0)
-      box_plane_update4<0, T, NOISE>(xc, row, zm, zp, g, st, coef, fb, xi, eg, 8 * gx, gx, ngx);
+      box_plane_update4<0, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
     else
-      box_plane_update4<1, T, NOISE>(xc, row, zm, zp, g, st, coef, fb, xi, eg, 8 * gx, gx, ngx);
+      box_plane_update4<1, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
     store8<T>(row, xc + 1);
   }
   __syncthreads();
@@ -911,6 +927,154 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     T v[8];
     load8<T>(P + (size_t)j * g.px + 8 * gx, v);
     store8<T>(x + base, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One whole 8-colour SOR/Gibbs direction of a 3D box-stencil (27-point) level
+// in a single pass: a workgroup owns a z-segment [ks, ke) of one chain and
+// marches over the planes of the parity swept first (P1: even planes going
+// forward, odd going backward). At P1 plane k it applies that plane's four
+// colours (neighbours: the old planes k +- 1), then the four colours of plane
+// k-1 (neighbours: the new planes k-2 and k). Four planes live in an LDS ring
+// (k-2 .. k+1); planes k+2, k+3 are loaded into registers while the eight
+// colour phases run. P1 planes just outside the segment are recomputed
+// (old inputs and vertex-keyed noise: the owner gets the same values) and
+// the sweep writes to a second buffer, so every workgroup reads old values
+// only. Result = the eight colour-phase launches of k_sweep. Rows are
+// parity-sorted as in k_sweep_box_planes; one thread per 8-vertex group.
+// ---------------------------------------------------------------------------
+// normals of the two colour phases of the thread's row on plane kz
+template <int NOISE>
+__device__ __forceinline__ void box_row_normals(int kz, const DevGeom& g, int chain, int j, int gx, int ngx,
+                                                bool active, uint32_t sample, const NoiseKey& nk, float (&xi)[2][4]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) xi[a][b] = 0.f;
+  if (NOISE != NOISE_PHILOX || !active) return;
+  for (int ipar = 0; ipar < 2; ++ipar) {
+    uint32_t c0 = (uint32_t)((((long long)kz * g.py + j) * ngx + gx) << 1) | (uint32_t)ipar;
+    normals4(c0, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xi[ipar]);
+  }
+}
+
+template <typename T, int NOISE>
+__device__ __forceinline__ void box_march_plane(T* __restrict__ R, size_t psz, int kz, const DevGeom& g,
+                                                const StencilT<T>& st, const T* __restrict__ coef, const T* fb,
+                                                const T* __restrict__ ext, long long cb, int j, int gx, int ngx,
+                                                bool active, int forward, const float (&xi)[2][4]) {
+  T* P = R + (size_t)(kz & 3) * psz;
+  const T* zm = R + (size_t)((kz - 1) & 3) * psz + (size_t)j * g.px + 8 * gx;
+  const T* zp = R + (size_t)((kz + 1) & 3) * psz + (size_t)j * g.px + 8 * gx;
+  const int jpar = j & 1;
+  const T* eg = NOISE == NOISE_EXTERNAL ? ext + cb + ((long long)kz * g.py + j) * g.px + 8 * gx : nullptr;
+  for (int ph = 0; ph < 4; ++ph) {
+    const int q = forward ? ph : 3 - ph;  // in-plane colour: ipar = q & 1, jpar = q >> 1
+    __syncthreads();
+    if (!active || jpar != (q >> 1)) continue;
+    const int ipar = q & 1;
+    T* row = P + (size_t)j * g.px + 8 * gx;
+    T xc[10];
+    load8<T>(row, xc + 1);
+    xc[0] = gx > 0 ? row[-1] : (T)0;
+    xc[9] = gx < ngx - 1 ? row[8] : (T)0;
+    if (ipar == 0)
+      box_plane_update4<0, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
+    else
+      box_plane_update4<1, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
+    store8<T>(row, xc + 1);
+  }
+}
+
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(1024) k_sweep_box_march(const T* __restrict__ x, T* __restrict__ xout,
+                                                          const T* __restrict__ f, const T* __restrict__ ext,
+                                                          DevGeom g, StencilT<T> st, const T* __restrict__ coef,
+                                                          int forward, int SZ, int nseg, int hth, NoiseKey nk) {
+  MGMC_DYNAMIC_SHARED(char, lds_raw);
+  T* R = reinterpret_cast<T*>(lds_raw);  // four plane slots, plane p in slot p & 3
+  const size_t psz = (size_t)g.py * g.px;
+  const int ngx = g.px >> 3;
+  const int tid = threadIdx.x;
+  const bool odd = tid < hth;
+  const int t2 = odd ? tid : tid - hth;
+  const int gx = t2 % ngx;
+  const int j = 2 * (t2 / ngx) + (odd ? 1 : 2);
+  const bool rowok = j <= g.n1 - 1;
+  const bool active = rowok && 8 * gx <= g.n0 - 1;
+  const int seg = blockIdx.x % nseg;
+  const int chain = blockIdx.x / nseg;
+  const int ks = 1 + seg * SZ;
+  const int ke = ks + SZ < g.n2 ? ks + SZ : g.n2;  // owned planes [ks, ke)
+  const int p1 = forward ? 0 : 1;                  // parity of the planes swept first
+  const long long cb = (long long)chain * g.cs;
+  const long long fcb = (long long)chain * g.fcs;
+  const uint32_t sample = *nk.sample;
+  // the thread's group of each plane it loads (one 8-vertex group: py * ngx <= threads)
+  const int lr = tid / ngx, lg = tid % ngx;
+  const bool loader = lr < g.py;
+  const long long loff = (long long)lr * g.px + 8 * lg;
+  int kA = ks - 1;
+  if ((kA & 1) != p1) ++kA;  // first P1 plane: ks - 1 or ks
+  int kB = ke;
+  if ((kB & 1) != p1) --kB;  // last P1 plane: ke or ke - 1
+  // initial ring: old planes kA - 1, kA, kA + 1
+  for (int d = -1; d <= 1; ++d) {
+    const int p = kA + d;
+    if (loader) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (T)0;
+      if (p >= 0 && p <= g.n2) load8<T>(x + cb + (long long)p * psz + loff, v);
+      store8<T>(R + (size_t)(p & 3) * psz + loff, v);
+    }
+  }
+  for (int k = kA; k <= kB; k += 2) {
+    // planes k+2, k+3 in flight during the colour phases of this step
+    T n2v[8], n3v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) n2v[q] = n3v[q] = (T)0;
+    const bool more = k + 2 <= kB;
+    if (loader && more) {
+      if (k + 2 <= g.n2) load8<T>(x + cb + (long long)(k + 2) * psz + loff, n2v);
+      if (k + 3 <= g.n2) load8<T>(x + cb + (long long)(k + 3) * psz + loff, n3v);
+    }
+    const bool up1 = k >= 1 && k <= g.n2 - 1;
+    const int kb = k - 1;
+    const bool up2 = kb >= ks && kb < ke && kb >= 1 && kb <= g.n2 - 1;
+    T f1[8], f2[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f1[q] = f2[q] = (T)0;
+    if (active && up1) load8<T>(f + fcb + ((long long)k * g.py + j) * g.px + 8 * gx, f1);
+    if (active && up2) load8<T>(f + fcb + ((long long)kb * g.py + j) * g.px + 8 * gx, f2);
+    // the normals of both planes while the loads are in flight
+    float xi1[2][4], xi2[2][4];
+    box_row_normals<NOISE>(k, g, chain, j, gx, ngx, active && up1, sample, nk, xi1);
+    box_row_normals<NOISE>(kb, g, chain, j, gx, ngx, active && up2, sample, nk, xi2);
+    if (up1) box_march_plane<T, NOISE>(R, psz, k, g, st, coef, f1, ext, cb, j, gx, ngx, active, forward, xi1);
+    if (up2) box_march_plane<T, NOISE>(R, psz, kb, g, st, coef, f2, ext, cb, j, gx, ngx, active, forward, xi2);
+    __syncthreads();
+    // owned planes of this step to the output buffer (whole rows, padding included)
+    if (rowok) {
+      const long long roff = (long long)j * g.px + 8 * gx;
+      if (k >= ks && k < ke) {
+        T v[8];
+        load8<T>(R + (size_t)(k & 3) * psz + roff, v);
+        store8<T>(xout + cb + (long long)k * psz + roff, v);
+      }
+      if (kb >= ks && kb < ke) {
+        T v[8];
+        load8<T>(R + (size_t)(kb & 3) * psz + roff, v);
+        store8<T>(xout + cb + (long long)kb * psz + roff, v);
+      }
+    }
+    __syncthreads();
+    // planes k-2 and k-1 are done: their slots take k+2 and k+3
+    if (loader && more) {
+      store8<T>(R + (size_t)((k + 2) & 3) * psz + loff, n2v);
+      store8<T>(R + (size_t)((k + 3) & 3) * psz + loff, n3v);
+    }
   }
 }
 

@@ -130,6 +130,10 @@ struct Common {
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
   int tune_rr_threads = 512;
+  int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
+  int tune_march_blocks = 512; // ... workgroups they aim for
+  int tune_march_min_planes = 4; // ... fewest planes per z-segment
+  int max_lds = 65536;        // dynamic LDS per workgroup the device allows
   int tune_planes = 1;        // plane-parity sweeps on 3D box levels
   int tune_planes_min = 1;    // ... when planes x chains reaches this many workgroups
   int tune_planes_stage3 = 1; // ... with planes k-1, k, k+1 staged in LDS when they fit
@@ -226,7 +230,7 @@ struct Engine : EngineBase {
       x[l].alloc(n, C.stream);
       f[l].alloc(n, C.stream);
       r[l].alloc(n, C.stream);
-      if (fused_ok(l)) x2[l].alloc(n, C.stream);
+      if (fused_ok(l) || march_ok(l)) x2[l].alloc(n, C.stream);
     }
     HIP_CHECK(hipStreamSynchronize(C.stream));
   }
@@ -300,7 +304,7 @@ struct Engine : EngineBase {
   }
   void retune() override {
     for (int l = 0; l < C.nlevel; ++l)
-      if (fused_ok(l) && !x2[l].p) x2[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
+      if ((fused_ok(l) || march_ok(l)) && !x2[l].p) x2[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
     HIP_CHECK(hipStreamSynchronize(C.stream));
   }
   // ---- plane-parity sweep of 3D box levels (k_sweep_box_planes) ----
@@ -313,6 +317,61 @@ struct Engine : EngineBase {
     t.noise_scale = (T)s.noise_scale;
     return t;
   }
+  // single-pass 8-colour sweep of 3D box levels (k_sweep_box_march)
+  struct MarchShape {
+    int SZ, nseg, hth, threads;
+    size_t lds;
+  };
+  MarchShape march_shape(int l) const {
+    const DevGeom& g = C.L[l]->g;
+    MarchShape m;
+    m.hth = plane_half_threads(g);
+    const int ngx = g.px >> 3;
+    m.threads = std::max(2 * m.hth, (g.py * ngx + 63) / 64 * 64);
+    m.lds = (size_t)4 * g.py * g.px * sizeof(T);
+    const int nplanes = g.n2 - 1;
+    const int want = (C.tune_march_blocks + C.K - 1) / C.K;
+    int nseg = std::max(1, std::min(want, std::max(1, nplanes / C.tune_march_min_planes)));
+    m.SZ = (nplanes + nseg - 1) / nseg;
+    m.nseg = (nplanes + m.SZ - 1) / m.SZ;
+    return m;
+  }
+  bool march_ok(int l) const {
+    const Level& Lv = *C.L[l];
+    if (!C.tune_march || Lv.star || Lv.g.dim != 3) return false;
+    const MarchShape m = march_shape(l);
+    return m.threads <= 1024 && m.lds <= (size_t)C.max_lds;
+  }
+  void march_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
+    Level& Lv = *C.L[l];
+    const MarchShape m = march_shape(l);
+    const StencilT<T> stt = stencil_t(Lv.st);
+    T* in = X(l);
+    T* out = cur[l] ? x[l].p : x2[l].p;
+    NoiseKey nk = key(tag);
+    dim3 grid((unsigned)(m.nseg * C.K));
+#define MGMC_MARCH(NOISE)                                                                                       \
+  do {                                                                                                          \
+    set_lds_limit(reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE>), m.lds);                           \
+    launch(grid, m.threads, m.lds, C.stream, k_sweep_box_march<T, NOISE>, in, out, f[l].p, extp, geo(l), stt,    \
+           coef[l].p, (int)forward, m.SZ, m.nseg, m.hth, nk);                                                  \
+  } while (0)
+    if (noise == NOISE_NONE) MGMC_MARCH(NOISE_NONE);
+    else if (noise == NOISE_PHILOX) MGMC_MARCH(NOISE_PHILOX);
+    else MGMC_MARCH(NOISE_EXTERNAL);
+#undef MGMC_MARCH
+    cur[l] ^= 1;
+  }
+  // dynamic LDS above 64 KB needs the kernel's attribute raised once
+  static void set_lds_limit(const void* kernel, size_t bytes) {
+#ifndef MGMC_EMULATION
+    if (bytes > 65536) HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+#else
+    (void)kernel;
+    (void)bytes;
+#endif
+  }
+
   bool planes_ok(int l) const {
     const Level& Lv = *C.L[l];
     if (!C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
@@ -461,6 +520,10 @@ struct Engine : EngineBase {
     for (int s = 0; s < sparse_repeats; ++s) {
       if (fused_ok(l)) {
         fused_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
+        continue;
+      }
+      if (march_ok(l)) {
+        march_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
         continue;
       }
       if (planes_ok(l)) {
@@ -842,6 +905,15 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     HIP_CHECK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     C.own_stream = true;
   }
+#ifndef MGMC_EMULATION
+  {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&C.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  }
+#else
+  C.max_lds = 163840;
+#endif
   for (int l = 0; l < C.nlevel; ++l) {
     const HostLevel& HL = C.H.levels[l];
     auto Lv = std::make_unique<Level>();
@@ -1319,6 +1391,14 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
   else if (k == "plane_sweep") h->C.tune_planes = value != 0;
   else if (k == "fold_prolongation") h->C.tune_fold_prolong = value != 0;
   else if (k == "fused_residual") h->C.tune_fused_rr = value != 0;
+  else if (k == "box_march") h->C.tune_march = value != 0;
+  else if (k == "box_march_blocks") {
+    if (value < 1) throw Error(MGMC_ERR_ARG, "box_march_blocks must be >= 1");
+    h->C.tune_march_blocks = value;
+  } else if (k == "box_march_min_planes") {
+    if (value < 1) throw Error(MGMC_ERR_ARG, "box_march_min_planes must be >= 1");
+    h->C.tune_march_min_planes = value;
+  }
   else if (k == "rr_threads") {
     if (value < 64 || value > 1024) throw Error(MGMC_ERR_ARG, "rr_threads must lie in [64, 1024]");
     h->C.tune_rr_threads = value;

@@ -214,8 +214,10 @@ def test_plane_sweep_matches_colour_phases(precision, tol, stage3):
     p = params(3)
     K = 2
     ref = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    ref.set_tuning("box_march", 0)
     ref.set_tuning("plane_sweep", 0)
     new = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    new.set_tuning("box_march", 0)
     new.set_tuning("plane_sweep_min", 1)
     new.set_tuning("plane_sweep_stage3", stage3)
     rng = np.random.default_rng(10)
@@ -272,3 +274,33 @@ def test_fused_residual_restriction_matches_oracle(n, threads):
         want = omg.restrict(0, f[c] - omg.level_apply(0, x[c]))
         assert _rel(got[c], want) < 1e-12
         assert _rel(got[c], sep[c]) < 1e-13
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+@pytest.mark.parametrize("n,blocks,min_planes", [((32, 32, 16), 512, 1), ((32, 32, 16), 512, 2),
+                                                  ((16, 48, 32), 6, 3), ((32, 32, 16), 1, 1)])
+def test_box_march_matches_colour_phases(precision, tol, n, blocks, min_planes):
+    """The single-pass 8-colour sweep of box levels (z-segments with
+    recomputed boundary planes, LDS ring of four planes) equals the eight
+    colour-phase launches: Philox and injected noise, both directions,
+    segments of one plane up to whole chains."""
+    op, _ = make_problem(3, n, 3)
+    p = params(3)
+    K = 3
+    ref = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=19)
+    ref.set_tuning("box_march", 0)
+    ref.set_tuning("plane_sweep", 0)
+    new = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=19)
+    new.set_tuning("box_march", 1)
+    new.set_tuning("box_march_blocks", blocks)
+    new.set_tuning("box_march_min_planes", min_planes)
+    rng = np.random.default_rng(21)
+    for level in (1, 2):
+        N = ref.ndof(level)
+        b, x, xi = rng.standard_normal((3, K, N))
+        xl = rng.standard_normal((K, 3))
+        for fw in (True, False):
+            for st, kw in ((False, {}), (True, {}), (True, {"xi": xi, "xi_lr": xl})):
+                a = ref.sweep(level, fw, st, b, x, **kw)
+                c = new.sweep(level, fw, st, b, x, **kw)
+                assert np.abs(a - c).max() <= tol * np.abs(a).max(), (level, fw, st, bool(kw))
