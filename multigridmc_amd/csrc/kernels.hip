@@ -372,7 +372,30 @@ __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long lo
   double s[LR_CPB];
 #pragma unroll
   for (int q = 0; q < LR_CPB; ++q) s[q] = 0.0;
-  for (int e = bptr[kcol] + threadIdx.x; e < bptr[kcol + 1]; e += blockDim.x) {
+  const int e1 = bptr[kcol + 1];
+  const int stride = blockDim.x;
+  int e = bptr[kcol] + threadIdx.x;
+  // four entries per trip: their index and value loads, then their x loads,
+  // are issued together (latency-bound gather)
+  for (; e + 3 * stride < e1; e += 4 * stride) {
+    long long ix[4];
+    double bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ix[u] = bidx[e + u * stride];
+      bv[u] = bval[e + u * stride];
+    }
+    T xv[4][LR_CPB];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < LR_CPB; ++q) xv[u][q] = q < nc ? x[(long long)(c0 + q) * cs + ix[u]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < LR_CPB; ++q) s[q] += bv[u] * (double)xv[u][q];
+  }
+  for (; e < e1; e += stride) {
     const double b = bval[e];
     const T* xp = x + (long long)c0 * cs + bidx[e];
 #pragma unroll
@@ -432,23 +455,36 @@ __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long 
   const int c0 = blockIdx.y * LR_CPU;
   const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
   const int tid = threadIdx.x;
+  const int r = blockIdx.x * blockDim.x + tid;
+  const bool ok = r < nrows;
+  // the row's x values and W entries are requested before the block waits
+  // for u in LDS (latency-bound scatter)
+  long long rr = 0;
+  int e0 = 0, e1 = 0;
+  if (ok) {
+    rr = rows[r];
+    e0 = ptr[r];
+    e1 = ptr[r + 1];
+  }
+  T xv[LR_CPU];
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
   for (int e = tid; e < nc * m; e += blockDim.x) us[e / m][e % m] = u[(long long)c0 * m + e];
   __syncthreads();
-  const int r = blockIdx.x * blockDim.x + tid;
-  if (r >= nrows) return;
+  if (!ok) return;
   double s[LR_CPU];
 #pragma unroll
   for (int q = 0; q < LR_CPU; ++q) s[q] = 0.0;
-  for (int e = ptr[r]; e < ptr[r + 1]; ++e) {
+  for (int e = e0; e < e1; ++e) {
     const double v = val[e];
     const int cl = col[e];
 #pragma unroll
     for (int q = 0; q < LR_CPU; ++q) s[q] += v * us[q][cl];
   }
-  T* p = x + (long long)c0 * cs + rows[r];
+  T* p = x + (long long)c0 * cs + rr;
 #pragma unroll
   for (int q = 0; q < LR_CPU; ++q)
-    if (q < nc) p[q * cs] = (T)((double)p[q * cs] + s[q]);
+    if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
 }
 
 // ---------------------------------------------------------------------------

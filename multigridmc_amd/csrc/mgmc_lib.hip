@@ -127,6 +127,7 @@ struct Common {
   int tune_fused = 1;
   int tune_rb_threads = 512;
   int tune_rb_blocks = 1024;
+  int tune_rb_lds_kb = 64;    // LDS budget of its tiles (KB; the device allows up to max_lds)
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
   int tune_rr_threads = 512;
@@ -245,7 +246,8 @@ struct Engine : EngineBase {
     const DevGeom& g = C.L[l]->g;
     int ngx = g.px >> 3;
     int by_threads = C.tune_rb_threads / ngx - 4;
-    int by_lds = (int)(((65536 - 40 * sizeof(T)) / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
+    const size_t budget = std::min((size_t)C.tune_rb_lds_kb * 1024, (size_t)C.max_lds);
+    int by_lds = (int)(((budget - 40 * sizeof(T)) / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
     return std::min(std::min(by_threads, by_lds), g.n1 - 1);
   }
   bool fused_ok(int l) const {
@@ -289,6 +291,11 @@ struct Engine : EngineBase {
       gc = C.L[l + 1]->g;
       alpha = (T)pending_alpha[l];
       pending[l] = 0;
+    }
+    if (s.lds > 65536) {
+      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE>), s.lds);
+      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX>), s.lds);
+      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL>), s.lds);
     }
     if (noise == NOISE_NONE)
       launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, geo(l), Lv.st, first,
@@ -1392,6 +1399,10 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
   else if (k == "fold_prolongation") h->C.tune_fold_prolong = value != 0;
   else if (k == "fused_residual") h->C.tune_fused_rr = value != 0;
   else if (k == "box_march") h->C.tune_march = value != 0;
+  else if (k == "rb_lds_kb") {
+    if (value < 16 || value > 160) throw Error(MGMC_ERR_ARG, "rb_lds_kb must lie in [16, 160]");
+    h->C.tune_rb_lds_kb = value;
+  }
   else if (k == "box_march_blocks") {
     if (value < 1) throw Error(MGMC_ERR_ARG, "box_march_blocks must be >= 1");
     h->C.tune_march_blocks = value;
