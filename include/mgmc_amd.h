@@ -182,6 +182,15 @@ int mgmc_measurement_vector(int dim, const int* n, const double* x0, double radi
  * CSR matrix from it and passing it to mgmc_create gives the FEM prior. */
 int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil);
 
+/* Adjoint contraction for parameter gradients (no counterpart in the
+ * reference): for every chain c, out[2c] = lam_c . mu_c and
+ * out[2c+1] = lam_c^T S mu_c over the fine interior vertices, S a constant
+ * stencil (27 entries in 3D, 9 in 2D, index as mgmc_level_stencil) applied
+ * with Dirichlet boundaries: the derivative of the prior operator with
+ * respect to a coefficient (e.g. the unit-diffusion stencil for a diffusion
+ * scale). lam, mu: nchains x N. */
+int mgmc_contract(mgmc_handle* h, const double* stencil, const double* lam, const double* mu, double* out);
+
 /* Export the level's stencil as used by the kernels (27 or 9 entries, x fastest,
  * offsets -1..1 per direction) and the colour count (2 = red-black). */
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);

@@ -1310,8 +1310,10 @@ __global__ void k_convert(T* __restrict__ y, const double* __restrict__ x, long 
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = (T)x[t];
 }
+// per-chain partial sums of squares: chain blockIdx.y, out[chain][block]
 __global__ void k_sumsq(const double* __restrict__ x, long long n, double* __restrict__ out) {
   double s = 0.0;
+  x += (long long)blockIdx.y * n;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
     s += x[t] * x[t];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -1319,7 +1321,69 @@ __global__ void k_sumsq(const double* __restrict__ x, long long n, double* __res
   int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) red[wid] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) out[(long long)blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Contraction of adjoint and forward fields for parameter gradients: for each
+// chain c, out[c][0] = sum_v lam_c[v] mu_c[v] and out[c][1] = sum_v lam_c[v]
+// (S mu_c)[v], S a constant 3^d stencil applied with Dirichlet boundaries (the
+// derivative of the operator with respect to a stencil parameter). Padded
+// FP64 fields; one 8-vertex group per thread, grid (row blocks, chains),
+// per-block partials out_part[c][block][2] summed on the host.
+__global__ void __launch_bounds__(256) k_contract(const double* __restrict__ lam, const double* __restrict__ mu,
+                                                  DevGeom g, StencilT<double> S, double* __restrict__ out_part) {
+  const int ngx = g.px >> 3;
+  const int nrows = (g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  double s0 = 0.0, s1 = 0.0;
+  if (t < ngx * nrows) {
+    const int gx = t % ngx, row = t / ngx;
+    const int j = row % (g.n1 - 1) + 1;
+    const int k = g.dim == 3 ? row / (g.n1 - 1) + 1 : 0;
+    const long long pl = (long long)g.px * g.py;
+    const long long base = (long long)c * g.cs + ((long long)k * g.py + j) * g.px + 8 * gx;
+    const int zr = g.dim == 3 ? 1 : 0;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int dz = -zr; dz <= zr; ++dz)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const double* p = mu + base + dz * pl + dy * g.px;
+        double w[10];
+        load8<double>(p, w + 1);
+        w[0] = p[-1];
+        w[9] = p[8];
+        const int cb = (g.dim == 3 ? (dz + 1) * 9 : 0) + (dy + 1) * 3;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += S.c[cb] * w[q] + S.c[cb + 1] * w[q + 1] + S.c[cb + 2] * w[q + 2];
+      }
+    double lv[8], mv[8];
+    load8<double>(lam + base, lv);
+    load8<double>(mu + base, mv);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = 8 * gx + q;
+      if (i >= 1 && i <= g.n0 - 1) {
+        s0 += lv[q] * mv[q];
+        s1 += lv[q] * acc[q];
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+  }
+  __shared__ double red[2][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = s0;
+    red[1][wid] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* o = out_part + ((long long)c * gridDim.x + blockIdx.x) * 2;
+    o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
 }
 
 }  // namespace mgmc

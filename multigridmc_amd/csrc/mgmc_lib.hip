@@ -1250,8 +1250,8 @@ int mgmc_apply_operator(mgmc_handle* h, const double* x, double* y) {
   API_END
 }
 
-// LoopSolver::apply (src/solver/loop_solver.cc:9-54) on all chains; the
-// stopping test and history use chain 0.
+// LoopSolver::apply (src/solver/loop_solver.cc:9-54) on all chains at once:
+// the stopping test must hold for every chain; hist records chain 0.
 int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
              int* niter) {
   API_BEGIN
@@ -1269,25 +1269,30 @@ int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double ato
   HIP_CHECK(hipMemsetAsync(h->b64.p, 0, np * sizeof(double), C.stream));
   double* db = stage(h->io1, b, n, C.stream);
   launch(nblocks(n), 256, 0, C.stream, k_pack<double>, db, h->b64.p, C.L[0]->g, C.K, 1);
-  const int nb = 512;
-  if (h->sums.n < (size_t)nb) h->sums.alloc(nb);
-  std::vector<double> part(nb);
+  const int nb = 128;
+  if (h->sums.n < (size_t)nb * C.K) h->sums.alloc((size_t)nb * C.K);
+  std::vector<double> part((size_t)nb * C.K);
   long long cs = C.L[0]->g.cs;
-  auto norm0 = [&](const double* v) {
-    launch(nb, 256, 0, C.stream, k_sumsq, v, cs, h->sums.p);
-    HIP_CHECK(hipMemcpyAsync(part.data(), h->sums.p, nb * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  auto norms = [&](const double* v) {  // per-chain 2-norms
+    launch(dim3(nb, C.K), 256, 0, C.stream, k_sumsq, v, cs, h->sums.p);
+    HIP_CHECK(hipMemcpyAsync(part.data(), h->sums.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
     sync(h);
-    double s = 0;
-    for (double v2 : part) s += v2;
-    return std::sqrt(s);
+    std::vector<double> out(C.K, 0.0);
+    for (int c = 0; c < C.K; ++c) {
+      for (int b2 = 0; b2 < nb; ++b2) out[c] += part[(size_t)c * nb + b2];
+      out[c] = std::sqrt(out[c]);
+    }
+    return out;
   };
-  double r0 = norm0(h->b64.p);
+  const std::vector<double> r0 = norms(h->b64.p);
   int it = -1;
   for (int k = 0; k < maxiter; ++k) {
     h->E->fine_residual_fp64(h->x64.p, h->b64.p, h->r64.p);  // r' = b - Q x
-    double rn = norm0(h->r64.p);
-    if (hist) hist[k] = rn;
-    if ((rn / r0 < rtol) && (rn < atol)) {
+    const std::vector<double> rn = norms(h->r64.p);
+    if (hist) hist[k] = rn[0];
+    bool done = true;
+    for (int c = 0; c < C.K; ++c) done = done && (rn[c] / r0[c] < rtol || r0[c] == 0.0) && rn[c] < atol;
+    if (done) {
       it = k;
       break;
     }
@@ -1301,6 +1306,49 @@ int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double ato
   HIP_CHECK(hipMemcpyAsync(x, db, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
   sync(h);
   if (it < 0) throw Error(MGMC_ERR_NOCONV, "solver failed to converge after maxiter iterations");
+  API_END
+}
+
+int mgmc_contract(mgmc_handle* h, const double* stencil, const double* lam, const double* mu, double* out) {
+  API_BEGIN
+  Common& C = h->C;
+  const DevGeom& g = C.L[0]->g;
+  int64_t N = C.L[0]->N;
+  size_t n = (size_t)N * C.K;
+  size_t np = (size_t)g.cs * C.K;
+  if (h->x64.n < np) {
+    h->x64.alloc(np, C.stream);
+    h->r64.alloc(np, C.stream);
+    h->b64.alloc(np, C.stream);
+  }
+  HIP_CHECK(hipMemsetAsync(h->x64.p, 0, np * sizeof(double), C.stream));
+  HIP_CHECK(hipMemsetAsync(h->b64.p, 0, np * sizeof(double), C.stream));
+  double* dl = stage(h->io1, lam, n, C.stream);
+  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, dl, h->b64.p, g, C.K, 1);
+  double* dm = stage(h->io2, mu, n, C.stream);
+  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, dm, h->x64.p, g, C.K, 1);
+  StencilT<double> S{};
+  for (int e = 0; e < 27; ++e) S.c[e] = 0.0;
+  const int ns = g.dim == 3 ? 27 : 9;
+  for (int e = 0; e < ns; ++e) S.c[e] = stencil[e];
+  const int ngx = g.px >> 3;
+  const int nrows = (g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1);
+  const unsigned nbk = nblocks((long long)ngx * nrows);
+  DevBuf<double> part;
+  part.alloc((size_t)nbk * C.K * 2);
+  launch(dim3(nbk, C.K), 256, 0, C.stream, k_contract, h->b64.p, h->x64.p, g, S, part.p);
+  std::vector<double> hp(part.n);
+  HIP_CHECK(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  for (int c = 0; c < C.K; ++c) {
+    double a = 0.0, b2 = 0.0;
+    for (unsigned k = 0; k < nbk; ++k) {
+      a += hp[((size_t)c * nbk + k) * 2];
+      b2 += hp[((size_t)c * nbk + k) * 2 + 1];
+    }
+    out[2 * c] = a;
+    out[2 * c + 1] = b2;
+  }
   API_END
 }
 

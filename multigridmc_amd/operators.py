@@ -78,9 +78,13 @@ class ShiftedLaplaceFDOperator:
 
     m_lowrank = 0
 
-    def __init__(self, lattice, kappa_sq):
+    def __init__(self, lattice, kappa_sq, diffusion=1.0):
+        """diffusion: a constant factor s on the Laplacian, h^d (-s Delta_h +
+        kappa^2) (1 in the reference; other values are passed to the library
+        as a CSR matrix)."""
         self.lattice = lattice
         self.kappa_sq = float(kappa_sq)
+        self.diffusion = float(diffusion)
 
     def get_ndof(self):
         return self.lattice.Nvertex
@@ -93,10 +97,26 @@ class ShiftedLaplaceFDOperator:
             h = 1.0 / v
             hinv2.append(1.0 / (h * h))
             cv *= h
+        s = self.diffusion
         diag = cv * self.kappa_sq
         for d in range(self.lattice.dim):
-            diag += 2.0 * cv * hinv2[d]
-        return diag, [-cv * h2 for h2 in hinv2]
+            diag += 2.0 * cv * (s * hinv2[d] if s != 1.0 else hinv2[d])
+        return diag, [-cv * (s * h2 if s != 1.0 else h2) for h2 in hinv2]
+
+    def laplacian_stencil(self):
+        """3^d stencil (library index order) of h^d (-Delta_h): the derivative
+        of the operator with respect to the diffusion factor."""
+        d = self.lattice.dim
+        cv = float(np.prod([1.0 / v for v in self.lattice.n]))
+        st = np.zeros(27 if d == 3 else 9)
+        centre = 13 if d == 3 else 4
+        strides = [1, 3, 9]
+        for k, v in enumerate(self.lattice.n):
+            hinv2 = float(v * v)
+            st[centre] += 2.0 * cv * hinv2
+            st[centre - strides[k]] -= cv * hinv2
+            st[centre + strides[k]] -= cv * hinv2
+        return st
 
     def get_sparse(self):
         L = self.lattice
@@ -266,7 +286,9 @@ class MeasuredOperator:
     measurement matrix and Sigma = variance_scaling * variance
     (src/linear_operator/measured_operator.cc:9-49)."""
 
-    def __init__(self, base, centres, radius, variance, variance_scaling=1.0):
+    def __init__(self, base, centres, radius, variance, variance_scaling=1.0, B=None):
+        """B: an already computed measurement matrix for these centres and
+        radius (reused when only the variances or the prior change)."""
         self.base = base
         self.lattice = base.lattice
         self.kappa_sq = base.kappa_sq
@@ -276,6 +298,9 @@ class MeasuredOperator:
         m = len(centres)
         self.m_lowrank = m
         self.Sigma = variance_scaling * np.asarray(variance, dtype=np.float64).reshape(m)
+        if B is not None:
+            self.B = B
+            return
         rows, cols, vals = [], [], []
         for k in range(m):
             r, v = measurement_vector(self.lattice, centres[k], radius)

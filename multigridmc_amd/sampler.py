@@ -45,7 +45,7 @@ def _problem(op):
     p.kappa_sq = op.kappa_sq
     keep = []
     prior = getattr(op, "base", op)
-    if not isinstance(prior, ShiftedLaplaceFDOperator):
+    if not isinstance(prior, ShiftedLaplaceFDOperator) or prior.diffusion != 1.0:
         A = prior.get_sparse().tocsr()
         A.sort_indices()
         rp, ci, v = i64(A.indptr), i64(A.indices), f64(A.data)
@@ -117,6 +117,14 @@ class _Handle:
         out = np.zeros_like(xx)
         check(_lib.lib().mgmc_apply_operator(self.h, ptr(xx), ptr(out)))
         return out.reshape(np.shape(x)) if np.size(x) == out.size else out
+
+    def contract(self, stencil, lam, mu):
+        """Per chain (lam . mu, lam^T S mu) for a constant stencil S
+        (mgmc_contract): the adjoint contraction of parameter gradients."""
+        ll, mm = self._rows(lam), self._rows(mu)
+        out = np.zeros((self.nchains, 2))
+        check(_lib.lib().mgmc_contract(self.h, ptr(f64(stencil)), ptr(ll), ptr(mm), ptr(out)))
+        return out
 
     def ndof(self, level):
         n = C.c_int64()
