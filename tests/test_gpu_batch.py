@@ -1,0 +1,59 @@
+"""A mixed batch of 16 posterior sampling problems on lattices drawn from
+{32^3, 48^3, 64^3, 96^3, 128^3} with 8-32 ball measurements each, run
+concurrently (one stream and one captured cycle per instance): every
+instance equals its own sequential run, and its observables have the exact
+posterior means and variances from FP64 multigrid solves."""
+import numpy as np
+import pytest
+
+import multigridmc_amd as mg
+from multigridmc_amd import measurements as ms
+from multigridmc_amd.batch import SamplerBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(seed=2024, count=16, nchains=4):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice([32, 48, 64, 96, 128]))
+        lat = mg.Lattice(n, n, n)
+        m = int(rng.integers(8, 33))
+        centres, y, var = mg.seeded_ball_measurements(lat, m, 0.05, 1e-4, seed + k)
+        op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
+        f = op.get_B() @ (y / var)
+        obs = [mg.measurement_vector(lat, rng.uniform(0.2, 0.8, 3), 0.1), mg.measurement_vector(lat, [0.5] * 3, 0.0)]
+        nlevel = ms._nlevel(op)
+        out.append(dict(op=op, params=mg.MultigridParameters(nlevel=nlevel), nchains=nchains, f=f,
+                        observables=obs, seed=77 + k, y=y))
+    return out
+
+
+def test_mixed_batch_equals_sequential_and_matches_posterior():
+    inst = _instances()
+    sizes = sorted({i["op"].lattice.n[0] for i in inst})
+    assert len(sizes) >= 3
+    batch = SamplerBatch(inst)
+    batch.run(30, record=False)
+    z = batch.run(300)
+    # the same instances one after the other (same seeds): identical samples
+    for k in (0, 5, 15):
+        i = inst[k]
+        s = mg.MultigridMCSampler(i["op"], i["params"], nchains=i["nchains"], precision="fp32", seed=i["seed"])
+        s.fix_rhs(i["f"])
+        s.set_state(np.zeros((i["nchains"], i["op"].lattice.Nvertex)))
+        s.set_observables(i["observables"])
+        s.run(30, record=False)
+        zs = s.run(300)
+        assert np.array_equal(zs, z[k]), k
+    # posterior moments of the observables, within Monte Carlo errors
+    for k, i in enumerate(inst):
+        means, cov = ms.observed_moments(i["op"], np.zeros(i["op"].lattice.Nvertex), i["y"], i["observables"],
+                                         rtol=1e-11)
+        for p in range(len(i["observables"])):
+            series = z[k][:, :, p]
+            tau = mg.tau_int_chains(series)[0]
+            se = np.sqrt(cov[p, p] * tau / series.size)
+            assert abs(series.mean() - means[p]) < 5 * se, (k, p)
+            assert abs(series.var() / cov[p, p] - 1.0) < 6 * np.sqrt(2 * tau / series.size) + 0.02, (k, p)
