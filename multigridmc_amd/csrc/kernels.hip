@@ -741,8 +741,15 @@ __global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __re
 // plane k before the first colour (TY+4 rows) and plane k-1 after it (TY+2
 // rows), each double-buffered so one barrier per step suffices.
 // ---------------------------------------------------------------------------
+// FP64 workgroups are capped at 512 threads, which gives the compiler 256
+// VGPRs for the plane window instead of spilling it.
+template <typename T>
+constexpr int rb3_max_threads() {
+  return sizeof(T) == 8 ? 512 : 1024;
+}
+
 template <typename T, int NOISE>
-__global__ void __launch_bounds__(1024) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
+__global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
                                                     NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,

@@ -245,7 +245,7 @@ struct Engine : EngineBase {
   int rb_ty(int l) const {
     const DevGeom& g = C.L[l]->g;
     int ngx = g.px >> 3;
-    int by_threads = C.tune_rb_threads / ngx - 4;
+    int by_threads = std::min(C.tune_rb_threads, rb3_max_threads<T>()) / ngx - 4;
     const size_t budget = std::min((size_t)C.tune_rb_lds_kb * 1024, (size_t)C.max_lds);
     int by_lds = (int)(((budget - 40 * sizeof(T)) / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
     return std::min(std::min(by_threads, by_lds), g.n1 - 1);
