@@ -1133,7 +1133,7 @@ __global__ void __launch_bounds__(1024) k_sweep_box_march(const T* __restrict__ 
 // done. One thread per fine 8-vertex group and fine row.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(1024) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
+__global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
                                                                   T* __restrict__ fc, DevGeom g, DevGeom gc,
                                                                   Stencil st, int TYc, int SZc, int ntile, int nseg) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);

@@ -585,7 +585,7 @@ struct Engine : EngineBase {
     const DevGeom& g = C.L[l]->g;
     const DevGeom& gc = C.L[l + 1]->g;
     int ngx = g.px >> 3;
-    int by_threads = (C.tune_rr_threads / ngx - 3) / 2;
+    int by_threads = (std::min(C.tune_rr_threads, rb3_max_threads<T>()) / ngx - 3) / 2;
     // LDS: (2TYc+3 + 2TYc+1) px + (2TYc+1) px/2 values <= 64 KB
     int by_lds = (int)(((65536 / sizeof(T)) / g.px * 2 - 9) / 10);
     int t = std::min(std::min(by_threads, by_lds), gc.n1 - 1);
