@@ -220,6 +220,7 @@ struct Engine : EngineBase {
     cur.assign(C.nlevel, 0);
     pending.assign(C.nlevel, 0);
     pending_alpha.assign(C.nlevel, 0.0);
+    raise_lds_limits();
     coef.resize(C.nlevel);
     for (int l = 0; l < C.nlevel; ++l) {
       T c27[27];
@@ -292,11 +293,6 @@ struct Engine : EngineBase {
       alpha = (T)pending_alpha[l];
       pending[l] = 0;
     }
-    if (s.lds > 65536) {
-      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE>), s.lds);
-      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX>), s.lds);
-      set_lds_limit(reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL>), s.lds);
-    }
     if (noise == NOISE_NONE)
       launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, geo(l), Lv.st, first,
              s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
@@ -358,24 +354,34 @@ struct Engine : EngineBase {
     NoiseKey nk = key(tag);
     dim3 grid((unsigned)(m.nseg * C.K));
 #define MGMC_MARCH(NOISE)                                                                                       \
-  do {                                                                                                          \
-    set_lds_limit(reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE>), m.lds);                           \
-    launch(grid, m.threads, m.lds, C.stream, k_sweep_box_march<T, NOISE>, in, out, f[l].p, extp, geo(l), stt,    \
-           coef[l].p, (int)forward, m.SZ, m.nseg, m.hth, nk);                                                  \
-  } while (0)
+  launch(grid, m.threads, m.lds, C.stream, k_sweep_box_march<T, NOISE>, in, out, f[l].p, extp, geo(l), stt,      \
+         coef[l].p, (int)forward, m.SZ, m.nseg, m.hth, nk)
     if (noise == NOISE_NONE) MGMC_MARCH(NOISE_NONE);
     else if (noise == NOISE_PHILOX) MGMC_MARCH(NOISE_PHILOX);
     else MGMC_MARCH(NOISE_EXTERNAL);
 #undef MGMC_MARCH
     cur[l] ^= 1;
   }
-  // dynamic LDS above 64 KB needs the kernel's attribute raised once
-  static void set_lds_limit(const void* kernel, size_t bytes) {
+  // dynamic LDS above 64 KB: the kernels that may use it get the device's
+  // limit as their attribute once, at set-up (not inside graph capture)
+  void raise_lds_limits() {
 #ifndef MGMC_EMULATION
-    if (bytes > 65536) HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-#else
-    (void)kernel;
-    (void)bytes;
+    if (C.max_lds <= 65536) return;
+    const void* ks[] = {
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE>),
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX>),
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL>),
+        reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_NONE>),
+        reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_PHILOX>),
+        reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_EXTERNAL>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_NONE, true>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_PHILOX, true>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_EXTERNAL, true>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_NONE, false>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_PHILOX, false>),
+        reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_EXTERNAL, false>),
+    };
+    for (const void* k : ks) HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, C.max_lds));
 #endif
   }
 
@@ -384,7 +390,7 @@ struct Engine : EngineBase {
     if (!C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
     const DevGeom& g = Lv.g;
     // one workgroup per (plane, chain): worth it only with enough workgroups
-    return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= 65536 &&
+    return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= (size_t)C.max_lds &&
            (long long)((g.n2 - 1) / 2) * C.K >= C.tune_planes_min;
   }
   void plane_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
@@ -394,7 +400,7 @@ struct Engine : EngineBase {
     const int threads = 2 * hth;
     const StencilT<T> stt = stencil_t(Lv.st);
     const size_t plane = (size_t)g.py * g.px * sizeof(T);
-    const bool stage3 = C.tune_planes_stage3 && 3 * plane <= 65536;
+    const bool stage3 = C.tune_planes_stage3 && 3 * plane <= (size_t)C.max_lds;
     const size_t lds = stage3 ? 3 * plane : plane;
     NoiseKey nk = key(tag);
     for (int h = 0; h < 2; ++h) {
