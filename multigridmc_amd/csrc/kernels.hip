@@ -487,6 +487,72 @@ __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long 
     if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
 }
 
+// k_lr_solve and k_lr_update in one launch: every block first forms u for
+// its LR_CPU chains (u_c = M (zc z_c + sqrt(Sigma) xi'_c), the same sums in
+// the same order as k_lr_solve, recomputed per row block: m^2 FMAs per chain),
+// then applies x_c[rows[r]] += (R u_c)_r.
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long long cs,
+                                                         const long long* __restrict__ rows,
+                                                         const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ col,
+                                                         const double* __restrict__ val, int nrows,
+                                                         const double* __restrict__ z, const double* __restrict__ M,
+                                                         const double* __restrict__ sqrt_sigma, double zc,
+                                                         const double* __restrict__ ext_lr, int m, int K, NoiseKey nk) {
+  __shared__ double a[LR_CPU][MMAX];
+  __shared__ double us[LR_CPU][MMAX];
+  const int c0 = blockIdx.y * LR_CPU;
+  const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
+  const int tid = threadIdx.x;
+  const int r = blockIdx.x * blockDim.x + tid;
+  const bool ok = r < nrows;
+  long long rr = 0;
+  int e0 = 0, e1 = 0;
+  if (ok) {
+    rr = rows[r];
+    e0 = ptr[r];
+    e1 = ptr[r + 1];
+  }
+  T xv[LR_CPU];
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
+  for (int e = tid; e < nc * m; e += blockDim.x) {
+    const int q = e / m, i = e % m, c = c0 + q;
+    double v = zc * z[(long long)c * m + i];
+    if (NOISE == NOISE_PHILOX) {
+      float xi[4];
+      normals4(LOWRANK_SLOT + (uint32_t)(i >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+      v += sqrt_sigma[i] * (double)xi[i & 3];
+    }
+    if (NOISE == NOISE_EXTERNAL) v += sqrt_sigma[i] * ext_lr[(long long)c * m + i];
+    a[q][i] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < nc * m; e += blockDim.x) {
+    const int q = e / m, i = e % m;
+    const double* Mi = M + i * m;
+    double acc = 0.0;
+    for (int l = 0; l < m; ++l) acc += Mi[l] * a[q][l];
+    us[q][i] = acc;
+  }
+  __syncthreads();
+  if (!ok) return;
+  double s[LR_CPU];
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q) s[q] = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    const double v = val[e];
+    const int cl = col[e];
+#pragma unroll
+    for (int q = 0; q < LR_CPU; ++q) s[q] += v * us[q][cl];
+  }
+  T* p = x + (long long)c0 * cs + rr;
+#pragma unroll
+  for (int q = 0; q < LR_CPU; ++q)
+    if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
+}
+
 // ---------------------------------------------------------------------------
 // Transfers. Grids are (groups x rows, chains): thread t of chain
 // blockIdx.y owns group t % ng of interior row t / ng, row id

@@ -502,10 +502,24 @@ struct Engine : EngineBase {
   }
 
   // u = M (a sqrt(Sigma) xi' - B_l^T x) for every chain (k_lr_dot + k_lr_solve)
+  // z = B_l^T x (k_lr_dot), u = M (a sqrt(Sigma) xi' - z) (k_lr_solve) and
+  // x[rows(R)] += R u (k_lr_update). With at most LR_CPU chains the solve runs
+  // inside the update launch instead (k_lr_solve_update: one launch less,
+  // u recomputed per row block); with more chains that redundancy costs more
+  // than the launch.
+  struct LrSolve {
+    const double* M = nullptr;
+    int noise = NOISE_NONE;
+    const double* extlr = nullptr;
+    uint32_t tag = 0;
+  } lr_next;
+  bool lr_fused() const { return C.K <= LR_CPU; }
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
     dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
     launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p);
+    lr_next = LrSolve{M, noise, extlr, tag};
+    if (lr_fused()) return;
     NoiseKey nk = key(tag);
     if (noise == NOISE_NONE)
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
@@ -516,12 +530,23 @@ struct Engine : EngineBase {
              nk);
   }
 
-  // x[rows(R)] += R u
+  // x[rows(R)] += R u with u from the last lr_dot_solve
   void lr_apply(T* xp, long long cs, DevRows& R) {
     if (R.nrows == 0) return;
     dim3 grid(nblocks(R.nrows), (C.K + LR_CPU - 1) / LR_CPU);
-    launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
-           C.K);
+    if (!lr_fused()) {
+      launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
+             C.K);
+      return;
+    }
+    NoiseKey nk = key(lr_next.tag);
+#define MGMC_LRSU(NOISE)                                                                                        \
+  launch(grid, 256, 0, C.stream, k_lr_solve_update<T, NOISE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p,      \
+         R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk)
+    if (lr_next.noise == NOISE_NONE) MGMC_LRSU(NOISE_NONE);
+    else if (lr_next.noise == NOISE_PHILOX) MGMC_LRSU(NOISE_PHILOX);
+    else MGMC_LRSU(NOISE_EXTERNAL);
+#undef MGMC_LRSU
   }
 
   // One directional SOR sweep (+ SMW), SORSmoother::apply / SORSampler::apply
