@@ -15,6 +15,9 @@
 // unknowns (and hence the triangular factor used in the SMW correction).
 //
 // The C ABI at the end of this file is what the Python tests bind via ctypes.
+// Each function is pinned by a check in tests/test_oracle_*.py against the
+// reference's known-answer tests, closed forms or brute force; none is left
+// unpinned.
 
 #include <algorithm>
 #include <array>

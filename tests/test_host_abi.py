@@ -105,3 +105,29 @@ def test_stats_estimators_on_ar1():
     chains = np.stack([x[i * 40000:(i + 1) * 40000] for i in range(10)], axis=1)
     tau_c, _ = mg.tau_int_chains(chains)
     assert abs(tau_c - tau_exact) < 0.15
+
+
+def test_setup_helpers_match_host_operators():
+    """mgmc_measurement_vector and mgmc_fem_stencil (C++ set-up code in the
+    library, no device work) equal the Python operators bit for bit."""
+    import ctypes as C
+    from multigridmc_amd import _lib
+    L = _lib.lib()
+    for dim, n, x0, r in [(3, (16, 16, 16), (0.3, 0.55, 0.61), 0.15), (2, (64, 32), (0.41, 0.52), 0.07),
+                          (3, (32, 32, 32), (0.2, 0.8, 0.45), 0.05), (3, (16, 16, 16), (0.3, 0.55, 0.61), 0.0)]:
+        idx, val = mg.measurement_vector(mg.Lattice(*n), x0, r)
+        nn = np.array(list(n) + [1] * (3 - len(n)), dtype=np.int32)
+        cap = 4096
+        oi, ov, nnz = np.zeros(cap, np.int64), np.zeros(cap), C.c_int64()
+        x0a = np.array(x0, dtype=np.float64)
+        _lib.check(L.mgmc_measurement_vector(dim, nn.ctypes.data, x0a.ctypes.data, r, cap, oi.ctypes.data,
+                                             ov.ctypes.data, C.byref(nnz)))
+        assert nnz.value == len(idx) and np.array_equal(oi[:len(idx)], idx) and np.array_equal(ov[:len(idx)], val)
+    for dim, n in [(2, (16, 32)), (3, (8, 16, 32))]:
+        st = np.zeros(27 if dim == 3 else 9)
+        nn = np.array(list(n) + [1] * (3 - len(n)), dtype=np.int32)
+        _lib.check(L.mgmc_fem_stencil(dim, nn.ctypes.data, 10.0, st.ctypes.data))
+        ref = mg.ShiftedLaplaceFEMOperator(mg.Lattice(*n), 10.0).stencil()
+        for s, v in ref.items():
+            e = ((s[2] + 1) * 9 if dim == 3 else 0) + (s[1] + 1) * 3 + (s[0] + 1)
+            assert abs(st[e] - v) <= 1e-15 * abs(v)
