@@ -41,6 +41,43 @@ __device__ __forceinline__ void store8(T* __restrict__ p, const T* v) {
   }
 }
 
+// LDS copy of an x-row of px = 8 ngx values, as 8 / C chunk planes of C =
+// 16 / sizeof(T) values per group: chunk h of group g sits at h C ngx + C g,
+// so the 16-byte accesses of the lanes of a row are consecutive (a plain
+// 8-value stride leaves half of the LDS banks idle and conflicts 2-way).
+template <typename T>
+struct LdsRow {
+  static constexpr int C = 16 / (int)sizeof(T), NCH = 8 / C;
+  __device__ static __forceinline__ void store(T* row, int gx, int ngx, const T* v) {
+#pragma unroll
+    for (int h = 0; h < NCH; ++h) {
+      T* q = row + h * C * ngx + C * gx;
+      if constexpr (sizeof(T) == 4)
+        *reinterpret_cast<float4*>(q) = make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
+      else
+        *reinterpret_cast<double2*>(q) = make_double2(v[2 * h], v[2 * h + 1]);
+    }
+  }
+  __device__ static __forceinline__ void load(const T* row, int gx, int ngx, T* v) {
+#pragma unroll
+    for (int h = 0; h < NCH; ++h) {
+      const T* q = row + h * C * ngx + C * gx;
+      if constexpr (sizeof(T) == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(q);
+        v[4 * h] = a.x; v[4 * h + 1] = a.y; v[4 * h + 2] = a.z; v[4 * h + 3] = a.w;
+      } else {
+        const double2 a = *reinterpret_cast<const double2*>(q);
+        v[2 * h] = a.x; v[2 * h + 1] = a.y;
+      }
+    }
+  }
+  // x(8 gx - 1) and x(8 gx + 8): the last value of group gx - 1, the first of gx + 1
+  __device__ static __forceinline__ T left(const T* row, int gx, int ngx) {
+    return row[(NCH - 1) * C * ngx + C * (gx - 1) + C - 1];
+  }
+  __device__ static __forceinline__ T right(const T* row, int gx, int ngx) { return row[C * (gx + 1)]; }
+};
+
 // Star-stencil update of the four vertices of parity IPAR in an 8-vertex
 // group: xc holds the group, `left`/`right` its outer neighbours, ym/yp/zm/zp
 // the same group in the neighbouring rows/planes, fb the right-hand side.
@@ -115,20 +152,20 @@ __device__ __forceinline__ void box_update4(T* xc, const T* xr, const DevGeom& g
 }
 
 // Box-stencil update of the four parity-IPAR vertices of a group whose own
-// plane is in LDS (`row` = the group in LDS, xc its values with margins);
-// zm / zp point at the same group in the planes below / above (LDS or
-// global memory). Outer margins of the first / last group are never used.
-template <int IPAR, typename T, int NOISE, bool ROLLED = (sizeof(T) == 8)>
+// plane is in LDS, xc its values with margins. CHUNK: `row`, `zm`, `zp` are
+// the starts of the group's row in LDS planes k, k-1, k+1 stored as LdsRow
+// chunks; otherwise they point at the group itself in plain rows (LDS or
+// global memory). Planes are visited one at a time (one plane's rows live:
+// 62 VGPRs in FP32, no spills in FP64); the centre row is read back from
+// LDS, where it still holds the values of this phase. Outer margins of the
+// first / last group are never used.
+template <int IPAR, typename T, int NOISE, bool CHUNK>
 __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* zm, const T* zp, const DevGeom& g,
                                                   const StencilT<T>& st, const T* __restrict__ coef, const T* fb,
                                                   const float* xi, const T* extg, int i0, int gx, int ngx) {
   T acc[4];
 #pragma unroll
   for (int l = 0; l < 4; ++l) acc[l] = (T)0;
-  if constexpr (ROLLED) {
-  // planes one at a time (keeps one plane's rows live: 62 instead of 124
-  // VGPRs in FP32, no spills in FP64); the centre row is read back from LDS,
-  // where it still holds the values of this phase
 #pragma unroll 1
   for (int dz = -1; dz <= 1; ++dz) {
     const T* pl = dz < 0 ? zm : dz > 0 ? zp : row;
@@ -137,9 +174,15 @@ __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* 
     for (int dy = -1; dy <= 1; ++dy) {
       T w[10];
       const T* p = pl + dy * g.px;
-      load8<T>(p, w + 1);
-      w[0] = gx > 0 ? p[-1] : (T)0;
-      w[9] = gx < ngx - 1 ? p[8] : (T)0;
+      if constexpr (CHUNK) {
+        LdsRow<T>::load(p, gx, ngx, w + 1);
+        w[0] = gx > 0 ? LdsRow<T>::left(p, gx, ngx) : (T)0;
+        w[9] = gx < ngx - 1 ? LdsRow<T>::right(p, gx, ngx) : (T)0;
+      } else {
+        load8<T>(p, w + 1);
+        w[0] = gx > 0 ? p[-1] : (T)0;
+        w[9] = gx < ngx - 1 ? p[8] : (T)0;
+      }
       const T a0 = cz[(dy + 1) * 3 + 0];
       const T a1 = cz[(dy + 1) * 3 + 1];
       const T a2 = cz[(dy + 1) * 3 + 2];
@@ -149,32 +192,6 @@ __device__ __forceinline__ void box_plane_update4(T* xc, const T* row, const T* 
         acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
       }
     }
-  }
-  } else {
-#pragma unroll
-  for (int dz = -1; dz <= 1; ++dz) {
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      T w[10];
-      if (dz == 0 && dy == 0) {
-#pragma unroll
-        for (int q = 0; q < 10; ++q) w[q] = xc[q];
-      } else {
-        const T* p = (dz < 0 ? zm : dz > 0 ? zp : row) + dy * g.px;
-        load8<T>(p, w + 1);
-        w[0] = gx > 0 ? p[-1] : (T)0;
-        w[9] = gx < ngx - 1 ? p[8] : (T)0;
-      }
-      const T a0 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 0];
-      const T a1 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 1];
-      const T a2 = st.c[(dz + 1) * 9 + (dy + 1) * 3 + 2];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const int q = 2 * l + IPAR;
-        acc[l] += a0 * w[q] + a1 * w[q + 1] + a2 * w[q + 2];
-      }
-    }
-  }
   }
   const T om_d = st.inv_diag_omega;
 #pragma unroll
@@ -874,8 +891,8 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
 
   const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
   const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
-  const size_t aoff = (size_t)r * g.px + 8 * gx;        // A rows j0-2 .. j0+TY+1
-  const size_t boff = (size_t)(r - 1) * g.px + 8 * gx;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
+  const size_t aoff = (size_t)r * g.px;        // A rows j0-2 .. j0+TY+1 (chunked, LdsRow)
+  const size_t boff = (size_t)(r - 1) * g.px;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
 
   // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
   // receives plane k+1; fm, fc = f on planes k-1, k
@@ -905,16 +922,16 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
     T* A = A0 + (k & 1) * asz;         // plane k before the first colour
     const T* Bm = B0 + (kb & 1) * bsz;  // plane k-1 after the first colour (written last step)
     T* Bk = B0 + (k & 1) * bsz;
-    if (active) store8<T>(A + aoff, xc);
+    if (active) LdsRow<T>::store(A + aoff, gx, ngx, xc);
     __syncthreads();
     // first colour on plane k (owned rows and halo ring)
     if (doA) {
       const T* Arow = A + aoff;
       T ym[8], yp[8];
-      load8<T>(Arow - g.px, ym);
-      load8<T>(Arow + g.px, yp);
-      const T left = gx > 0 ? Arow[-1] : (T)0;
-      const T right = gx < ngx - 1 ? Arow[8] : (T)0;
+      LdsRow<T>::load(Arow - g.px, gx, ngx, ym);
+      LdsRow<T>::load(Arow + g.px, gx, ngx, yp);
+      const T left = gx > 0 ? LdsRow<T>::left(Arow, gx, ngx) : (T)0;
+      const T right = gx < ngx - 1 ? LdsRow<T>::right(Arow, gx, ngx) : (T)0;
       const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)k * pl : nullptr;
       if (iparA == 0)
         star_update4<0, T, NOISE>(xc, left, right, ym, yp, xm, xp, fc, xiA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
@@ -925,10 +942,10 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
     if (doB) {
       const T* Brow = Bm + boff;
       T ym[8], yp[8];
-      load8<T>(Brow - g.px, ym);
-      load8<T>(Brow + g.px, yp);
-      const T left = gx > 0 ? Brow[-1] : (T)0;
-      const T right = gx < ngx - 1 ? Brow[8] : (T)0;
+      LdsRow<T>::load(Brow - g.px, gx, ngx, ym);
+      LdsRow<T>::load(Brow + g.px, gx, ngx, yp);
+      const T left = gx > 0 ? LdsRow<T>::left(Brow, gx, ngx) : (T)0;
+      const T right = gx < ngx - 1 ? LdsRow<T>::right(Brow, gx, ngx) : (T)0;
       const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)kb * pl : nullptr;
       if (iparB == 0)
         star_update4<0, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
@@ -937,7 +954,7 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
     }
     if (storeB) store8<T>(xout + cbase + (long long)kb * pl, xm);
     // plane k after the first colour, for the second colour at the next step
-    if (active && r >= 1 && r <= TY + 2) store8<T>(Bk + boff, xc);
+    if (active && r >= 1 && r <= TY + 2) LdsRow<T>::store(Bk + boff, gx, ngx, xc);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       xmm[q] = xm[q];
@@ -965,7 +982,9 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
                                                            const T* __restrict__ coef, int kpar, int forward,
                                                            int nplanes, int hth, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
-  // STAGE3: planes k-1, k, k+1 in LDS (py x px each); else plane k only
+  // STAGE3: planes k-1, k, k+1 in LDS (py x px each, LdsRow chunks); else
+  // plane k only (plain rows: planes k +- 1 are read from global memory)
+  constexpr bool CH = STAGE3;
   const size_t psz = (size_t)g.py * g.px;
   T* P = reinterpret_cast<T*>(lds_raw) + (STAGE3 ? psz : 0);
   const int ngx = g.px >> 3;
@@ -1001,15 +1020,20 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     }
   if (one) {
     if (tid < ngrp)
-      for (int d = 0; d < (STAGE3 ? 3 : 1); ++d)
-        store8<T>(P + (d - (STAGE3 ? 1 : 0)) * (long long)psz + (size_t)lr * g.px + 8 * lg, v[d]);
+      for (int d = 0; d < (STAGE3 ? 3 : 1); ++d) {
+        T* q = P + (d - (STAGE3 ? 1 : 0)) * (long long)psz + (size_t)lr * g.px;
+        if (CH) LdsRow<T>::store(q, lg, ngx, v[d]);
+        else store8<T>(q + 8 * lg, v[d]);
+      }
   } else {
     for (int dz = STAGE3 ? -1 : 0; dz <= (STAGE3 ? 1 : 0); ++dz)
       for (int e = tid; e < ngrp; e += blockDim.x) {
         const int rr = e / ngx, gg = e % ngx;
         T w[8];
         load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, w);
-        store8<T>(P + dz * (long long)psz + (size_t)rr * g.px + 8 * gg, w);
+        T* q = P + dz * (long long)psz + (size_t)rr * g.px;
+        if (CH) LdsRow<T>::store(q, gg, ngx, w);
+        else store8<T>(q + 8 * gg, w);
       }
   }
   const T* eg = NOISE == NOISE_EXTERNAL ? ext + base : nullptr;
@@ -1018,23 +1042,31 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     __syncthreads();
     if (!active || jpar != (q >> 1) || 8 * gx > g.n0 - 1) continue;
     const int ipar = q & 1;
-    T* row = P + (size_t)j * g.px + 8 * gx;
+    T* row = P + (size_t)j * g.px + (CH ? 0 : 8 * gx);
     T xc[10];
-    load8<T>(row, xc + 1);
-    xc[0] = gx > 0 ? row[-1] : (T)0;
-    xc[9] = gx < ngx - 1 ? row[8] : (T)0;
+    if (CH) {
+      LdsRow<T>::load(row, gx, ngx, xc + 1);
+      xc[0] = gx > 0 ? LdsRow<T>::left(row, gx, ngx) : (T)0;
+      xc[9] = gx < ngx - 1 ? LdsRow<T>::right(row, gx, ngx) : (T)0;
+    } else {
+      load8<T>(row, xc + 1);
+      xc[0] = gx > 0 ? row[-1] : (T)0;
+      xc[9] = gx < ngx - 1 ? row[8] : (T)0;
+    }
     const T* zm = STAGE3 ? row - psz : x + base - (long long)g.px * g.py;
     const T* zp = STAGE3 ? row + psz : x + base + (long long)g.px * g.py;
     if (ipar == 0)
-      box_plane_update4<0, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
+      box_plane_update4<0, T, NOISE, CH>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
     else
-      box_plane_update4<1, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
-    store8<T>(row, xc + 1);
+      box_plane_update4<1, T, NOISE, CH>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
+    if (CH) LdsRow<T>::store(row, gx, ngx, xc + 1);
+    else store8<T>(row, xc + 1);
   }
   __syncthreads();
   if (active) {
     T v[8];
-    load8<T>(P + (size_t)j * g.px + 8 * gx, v);
+    if (CH) LdsRow<T>::load(P + (size_t)j * g.px, gx, ngx, v);
+    else load8<T>(P + (size_t)j * g.px + 8 * gx, v);
     store8<T>(x + base, v);
   }
 }
@@ -1089,9 +1121,9 @@ __device__ __forceinline__ void box_march_plane(T* __restrict__ R, size_t psz, i
     xc[0] = gx > 0 ? row[-1] : (T)0;
     xc[9] = gx < ngx - 1 ? row[8] : (T)0;
     if (ipar == 0)
-      box_plane_update4<0, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
+      box_plane_update4<0, T, NOISE, false>(xc, row, zm, zp, g, st, coef, fb, xi[0], eg, 8 * gx, gx, ngx);
     else
-      box_plane_update4<1, T, NOISE, true>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
+      box_plane_update4<1, T, NOISE, false>(xc, row, zm, zp, g, st, coef, fb, xi[1], eg, 8 * gx, gx, ngx);
     store8<T>(row, xc + 1);
   }
 }

@@ -46,6 +46,16 @@ MGMC_HD void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
   lo = (uint32_t)p;
 }
 
+// a ^ b ^ c (one v_bitop3_b32 on the device: the compiler does not fuse
+// the two XORs of a Philox round by itself)
+MGMC_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 MGMC_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
@@ -56,8 +66,8 @@ MGMC_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, 
     uint32_t hi0, lo0, hi1, lo1;
     mulhilo(PHILOX_M0, c0, hi0, lo0);
     mulhilo(PHILOX_M1, c2, hi1, lo1);
-    uint32_t n0 = hi1 ^ c1 ^ k0;
-    uint32_t n2 = hi0 ^ c3 ^ k1;
+    uint32_t n0 = xor3(hi1, c1, k0);
+    uint32_t n2 = xor3(hi0, c3, k1);
     c0 = n0;
     c1 = lo1;
     c2 = n2;

@@ -33,7 +33,7 @@ Geom Geom::make(int dim, const int* n) {
   for (int d = 0; d < 3; ++d) g.n[d] = (d < dim) ? n[d] : 1;
   for (int d = 0; d < dim; ++d)
     if (g.n[d] < 2) throw Error(MGMC_ERR_ARG, "lattice extent must be >= 2 cells");
-  g.px = (int)round_up(g.n[0] + 1, 8);
+  g.px = (int)round_up(g.n[0], 8);
   g.py = g.n[1] + 1;
   g.pz = dim == 3 ? g.n[2] + 1 : 1;
   g.vol = (int64_t)g.px * g.py * g.pz;

@@ -3,11 +3,16 @@
 // Device data layout. Level l of a lattice with n_d cells per direction keeps
 // every field as a zero-padded box that includes the Dirichlet boundary
 // vertices: index (k * py + j) * px + i with 0 <= i <= n_x etc., pitch
-// px = round_up(n_x + 1, 8) (aligned 8-vertex groups for the noise and the
-// vector loads), py = n_y + 1, pz = n_z + 1 (1 in 2D). Boundary and padding
-// entries stay zero, so a constant stencil applied everywhere equals the
-// reference's operator with eliminated boundary rows. K chains are stored
-// back to back with a stride rounded up to 64 elements.
+// px = round_up(n_x, 8) (aligned 8-vertex groups for the noise and the
+// vector loads), py = n_y + 1, pz = n_z + 1 (1 in 2D). When n_x is a
+// multiple of 8 the right boundary vertex i = n_x of row j shares its slot
+// with the left boundary vertex i = 0 of row j + 1 (both are zero), so a row
+// holds no padding at all. Boundary and padding entries stay zero, so a
+// constant stencil applied everywhere equals the reference's operator with
+// eliminated boundary rows; kernels write boundary slots only with zero.
+// K chains are stored back to back with a stride rounded up to 64 elements;
+// the slot after the last vertex of a chain is the next chain's first
+// (boundary) slot, inter-chain padding or the field's trailing zero guard.
 #ifndef MGMC_SETUP_H
 #define MGMC_SETUP_H
 

@@ -62,7 +62,7 @@ def normals_reference(lat, nchains, tag, sample, seed):
     slot layout: ctr0 = ((row * px/8 + i/8) << 1) | (i & 1), lane (i & 7) >> 1."""
     n = lat.n
     dim = lat.dim
-    px = ((n[0] + 1 + 7) // 8) * 8
+    px = ((n[0] + 7) // 8) * 8
     py = n[1] + 1
     N = lat.Nvertex
     idx = lat.vertex_euclidean(np.arange(N))
