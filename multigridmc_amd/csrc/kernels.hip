@@ -980,7 +980,7 @@ template <typename T, int NOISE, bool STAGE3>
 __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
                                                            const T* __restrict__ ext, DevGeom g, StencilT<T> st,
                                                            const T* __restrict__ coef, int kpar, int forward,
-                                                           int nplanes, int hth, NoiseKey nk) {
+                                                           int nplanes, int hth, int xzero, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
   // STAGE3: planes k-1, k, k+1 in LDS (py x px each, LdsRow chunks); else
   // plane k only (plain rows: planes k +- 1 are read from global memory)
@@ -1003,13 +1003,23 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
   // stage plane k (and k +- 1), boundary rows 0 and n1 included, in LDS: the
   // loads are issued first and the normals of the thread's two colour phases
   // are drawn while they are in flight
+  // xzero (STAGE3 only): 1 = every plane of x is zero, 2 = the planes of
+  // parity kpar are zero; zero planes are not read
   const int ngrp = g.py * ngx;
   const bool one = ngrp <= (int)blockDim.x;  // one group per thread and plane
   T v[STAGE3 ? 3 : 1][8];
   const int lr = tid / ngx, lg = tid % ngx;
   if (one && tid < ngrp)
-    for (int d = 0; d < (STAGE3 ? 3 : 1); ++d)
-      load8<T>(x + (long long)chain * g.cs + ((long long)(k + d - (STAGE3 ? 1 : 0)) * g.py + lr) * g.px + 8 * lg, v[d]);
+    for (int d = 0; d < (STAGE3 ? 3 : 1); ++d) {
+      const bool own = !STAGE3 || d == 1;
+      if (xzero == 1 || (xzero == 2 && own)) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[d][q] = (T)0;
+      } else {
+        load8<T>(x + (long long)chain * g.cs + ((long long)(k + d - (STAGE3 ? 1 : 0)) * g.py + lr) * g.px + 8 * lg,
+                 v[d]);
+      }
+    }
   T fb[8];
   if (active) load8<T>(f + base + (long long)chain * (g.fcs - g.cs), fb);
   float xi[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
@@ -1030,7 +1040,12 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
       for (int e = tid; e < ngrp; e += blockDim.x) {
         const int rr = e / ngx, gg = e % ngx;
         T w[8];
-        load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, w);
+        if (xzero == 1 || (xzero == 2 && dz == 0)) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) w[q] = (T)0;
+        } else {
+          load8<T>(x + (long long)chain * g.cs + ((long long)(k + dz) * g.py + rr) * g.px + 8 * gg, w);
+        }
         T* q = P + dz * (long long)psz + (size_t)rr * g.px;
         if (CH) LdsRow<T>::store(q, gg, ngx, w);
         else store8<T>(q + 8 * gg, w);

@@ -200,7 +200,17 @@ struct Engine : EngineBase {
   std::vector<DevBuf<T>> coef;  // the level's 27 stencil coefficients in precision T
   Field<T> ext;  // external noise staging (test hooks)
 
-  T* X(int l) { return cur[l] ? x2[l].p : x[l].p; }
+  // x of level l is logically zero but not yet written (zero_x_lazy): the
+  // first plane-parity sweep or the coarse sample absorbs it; every other
+  // access through X() clears the field first
+  std::vector<int> xzero;
+  T* X(int l) {
+    if (xzero[l]) {
+      xzero[l] = 0;
+      zero_x(l);
+    }
+    return cur[l] ? x2[l].p : x[l].p;
+  }
   // Level geometry as the kernels see it: when every chain was given the same
   // right-hand side on level 0 (the sampler's f), the level-0 kernels read
   // chain 0's copy for all chains (fcs = 0), so f costs one read per level
@@ -218,6 +228,7 @@ struct Engine : EngineBase {
     r.resize(C.nlevel);
     x2.resize(C.nlevel);
     cur.assign(C.nlevel, 0);
+    xzero.assign(C.nlevel, 0);
     pending.assign(C.nlevel, 0);
     pending_alpha.assign(C.nlevel, 0.0);
     raise_lds_limits();
@@ -393,6 +404,11 @@ struct Engine : EngineBase {
     return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= (size_t)C.max_lds &&
            (long long)((g.n2 - 1) / 2) * C.K >= C.tune_planes_min;
   }
+  // planes k - 1, k, k + 1 of a plane-parity workgroup fit in LDS
+  bool planes_stage3(int l) const {
+    const DevGeom& g = C.L[l]->g;
+    return planes_ok(l) && C.tune_planes_stage3 && 3 * (size_t)g.py * g.px * sizeof(T) <= (size_t)C.max_lds;
+  }
   void plane_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
     Level& Lv = *C.L[l];
     const DevGeom& g = Lv.g;
@@ -400,17 +416,23 @@ struct Engine : EngineBase {
     const int threads = 2 * hth;
     const StencilT<T> stt = stencil_t(Lv.st);
     const size_t plane = (size_t)g.py * g.px * sizeof(T);
-    const bool stage3 = C.tune_planes_stage3 && 3 * plane <= (size_t)C.max_lds;
+    const bool stage3 = planes_stage3(l);
     const size_t lds = stage3 ? 3 * plane : plane;
+    // a lazily zeroed x (zero_x_lazy) is absorbed here: the first launch
+    // reads no plane of x, the second none of its own planes
+    const bool xz = stage3 && xzero[l];
+    if (xz) xzero[l] = 0;
+    T* xl = X(l);
     NoiseKey nk = key(tag);
     for (int h = 0; h < 2; ++h) {
       const int kpar = forward ? h : 1 - h;
       const int nplanes = kpar ? g.n2 / 2 : (g.n2 - 1) / 2;
       if (nplanes <= 0) continue;
       dim3 grid((unsigned)(nplanes * C.K));
+      const int xzf = xz ? (h == 0 ? 1 : 2) : 0;
 #define MGMC_PLANES(NOISE, S3)                                                                                  \
-  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, X(l), f[l].p, extp, geo(l), stt, coef[l].p, \
-         kpar, (int)forward, nplanes, hth, nk)
+  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, xl, f[l].p, extp, geo(l), stt, coef[l].p, \
+         kpar, (int)forward, nplanes, hth, xzf, nk)
       if (stage3) {
         if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, true);
         else if (noise == NOISE_PHILOX) MGMC_PLANES(NOISE_PHILOX, true);
@@ -700,6 +722,7 @@ struct Engine : EngineBase {
                   "preconditioner always solves it with Cholesky); add levels");
     int l = C.nlevel - 1;
     Level& Lv = *C.L[l];
+    xzero[l] = 0;  // k_coarse writes every interior value and reads no x
     size_t sh = 2 * (size_t)C.Nc * sizeof(double);
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
@@ -715,7 +738,24 @@ struct Engine : EngineBase {
                                                               C.Qinv.p, (int)C.Nc, xi_dev, nk);
   }
 
-  void zero_x(int l) { fill(X(l), (long long)C.L[l]->g.cs * C.K, (T)0); }
+  void zero_x(int l) { fill(cur[l] ? x2[l].p : x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0); }
+  // x_l = 0 at the start of a coarse-level visit. Boundary slots, padding and
+  // guards are zero from allocation and no kernel writes them with anything
+  // else, so a consumer that writes every interior value without reading x
+  // can take the place of the fill: k_coarse (the Cholesky sample / solve on
+  // the coarsest level) and the plane-parity sweep, which then skips its x
+  // loads (first launch: every plane zero; second launch: its own planes).
+  void zero_x_lazy(int l) {
+    if (l == C.nlevel - 1 && C.p.coarse_solver == MGMC_COARSE_CHOLESKY && C.Nc > 0) {
+      xzero[l] = 1;
+      return;
+    }
+    if (planes_stage3(l) && !fused_ok(l) && !march_ok(l)) {
+      xzero[l] = 1;
+      return;
+    }
+    zero_x(l);
+  }
 
   // MultigridMCSampler::sample (multigridmc_sampler.cc:109-136)
   void sample(int l) {
@@ -732,7 +772,7 @@ struct Engine : EngineBase {
     for (int j = 0; j < cyc; ++j) {
       pre(l, true);
       residual_restrict(l);
-      zero_x(l + 1);
+      zero_x_lazy(l + 1);
       sample(l + 1);
       prolongate_or_defer(l, C.p.coarse_scaling, C.p.npostsmooth > 0);
       post(l, true);
@@ -741,7 +781,8 @@ struct Engine : EngineBase {
 
   // MultigridPreconditioner::solve (multigrid_preconditioner.cc:74-101)
   void solve(int l) {
-    zero_x(l);
+    if (l > 0) zero_x_lazy(l);
+    else zero_x(l);
     if (l == C.nlevel - 1) {
       coarse(NOISE_NONE, nullptr);
       return;
