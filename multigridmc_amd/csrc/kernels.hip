@@ -1288,23 +1288,23 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star
   }
   if (rvalid) load8<T>(f + fbase + (long long)p0 * pl, fb);
   T acc[4] = {(T)0, (T)0, (T)0, (T)0};
-  T* LXrow = LX + (size_t)rt * g.px + 8 * gx;
-  T* LRrow = LR + (size_t)(rt - 1) * g.px + 8 * gx;
+  T* LXrow = LX + (size_t)rt * g.px;  // LdsRow chunks
+  T* LRrow = LR + (size_t)(rt - 1) * g.px;
   T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
 
   for (int p = p0; p <= p1; ++p) {
     // planes p + 1 of x and f are in flight while plane p is processed
     if (has_mem) load8<T>(x + cbase + (long long)(p + 1) * pl, xp);
     if (rvalid && p < p1) load8<T>(f + fbase + (long long)(p + 1) * pl, fn);
-    if (active) store8<T>(LXrow, xc);
+    if (active) LdsRow<T>::store(LXrow, gx, ngx, xc);
     __syncthreads();
     T r[8];
     if (rrow) {
       T ym[8], yp[8];
-      load8<T>(LXrow - g.px, ym);
-      load8<T>(LXrow + g.px, yp);
-      const T left = gx > 0 ? LXrow[-1] : (T)0;
-      const T right = gx < ngx - 1 ? LXrow[8] : (T)0;
+      LdsRow<T>::load(LXrow - g.px, gx, ngx, ym);
+      LdsRow<T>::load(LXrow + g.px, gx, ngx, yp);
+      const T left = gx > 0 ? LdsRow<T>::left(LXrow, gx, ngx) : (T)0;
+      const T right = gx < ngx - 1 ? LdsRow<T>::right(LXrow, gx, ngx) : (T)0;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const T xl = q > 0 ? xc[q - 1] : left;
@@ -1314,11 +1314,11 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star
         const int i = 8 * gx + q;
         r[q] = (rvalid && i >= 1 && i <= g.n0 - 1) ? fb[q] - a : (T)0;
       }
-      store8<T>(LRrow, r);
+      LdsRow<T>::store(LRrow, gx, ngx, r);
     }
     __syncthreads();
     if (rrow) {
-      const T rl = gx > 0 ? LRrow[-1] : (T)0;
+      const T rl = gx > 0 ? LdsRow<T>::left(LRrow, gx, ngx) : (T)0;
       T t[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) t[a] = half * (a > 0 ? r[2 * a - 1] : rl) + r[2 * a] + half * r[2 * a + 1];
