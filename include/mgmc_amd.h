@@ -197,8 +197,9 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
 
 /* Launch-shape tuning (no reference counterpart). Keys: "fused_sweep" (0/1:
  * sweep 3D star levels with the single-pass red-black kernel), "rb_threads"
- * (threads per workgroup of that kernel, 64..1024), "rb_blocks" (workgroups
- * it aims for; more z-segments when fewer tiles x chains exist),
+ * (threads per workgroup of that kernel, 64..1024; default 512), "rb_blocks"
+ * (workgroups it and the fused residual-restriction aim for, default 768;
+ * more z-segments when fewer tiles x chains exist),
  * "fold_prolongation" (0/1: add the coarse correction inside the first
  * post-smoothing sweep), "fused_residual" (0/1: residual + restriction in one
  * pass on 3D star levels), "rr_threads" (its threads per workgroup,

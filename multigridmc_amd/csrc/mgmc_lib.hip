@@ -126,7 +126,7 @@ struct Common {
   // and the number of workgroups it aims for
   int tune_fused = 1;
   int tune_rb_threads = 512;
-  int tune_rb_blocks = 1024;
+  int tune_rb_blocks = 768;  // 128^3 x 64 chains: three z-segments (longer marches beat fuller last waves)
   int tune_rb_lds_kb = 64;    // LDS budget of its tiles (KB; the device allows up to max_lds)
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
