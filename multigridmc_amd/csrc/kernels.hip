@@ -767,43 +767,73 @@ __global__ void __launch_bounds__(256) k_unpack(const T* __restrict__ pad, doubl
 // Coarsest level: x = L^{-T} xi + Q^{-1} f (= L^{-T}(xi + L^{-1} f),
 // src/sampler/cholesky_sampler.hh:52-66) or x = Q^{-1} f
 // (src/solver/cholesky_solver.cc:30-40); FP64 inside. Grid (row blocks of 64,
-// chains); the four waves of a block split the inner dimension and reduce
-// through LDS. The matrices are stored column-major (element (r, j) at
-// j * Nc + r) so that a wave reads 64 consecutive rows of one column.
-template <typename T, int NOISE>
-__global__ void __launch_bounds__(256) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
-                                                const long long* __restrict__ pidx, const double* __restrict__ LT,
-                                                const double* __restrict__ Q, int Nc, const double* __restrict__ ext,
-                                                NoiseKey nk) {
+// groups of CPT chains); the P = blockDim / 64 waves of a block split the
+// inner dimension and reduce through LDS in a fixed order, and every matrix
+// element a thread loads serves the CPT chains of the block (right-hand
+// sides and normals staged in LDS). The matrices are stored column-major
+// (element (r, j) at j * Nc + r) so that a wave reads 64 consecutive rows of
+// one column. LDS: max(2 CPT Nc, P CPT 64) doubles.
+template <typename T, int NOISE, int CPT>
+__global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
+                                                 const long long* __restrict__ pidx, const double* __restrict__ LT,
+                                                 const double* __restrict__ Q, int Nc, int K,
+                                                 const double* __restrict__ ext, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(double, sh);
-  double* fs = sh;
-  double* xs = sh + Nc;
-  __shared__ double red[4][64];
-  const int c = blockIdx.y;
+  double* fs = sh;             // [CPT][Nc]
+  double* xs = sh + CPT * Nc;  // [CPT][Nc]
+  const int c0 = blockIdx.y * CPT;
   const int tid = threadIdx.x;
-  for (int j = tid; j < Nc; j += blockDim.x) {
-    fs[j] = (double)f[c * fcs + pidx[j]];
-    if (NOISE == NOISE_PHILOX) {
-      float xi[4];
-      normals4(COARSE_SLOT + (uint32_t)(j >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
-      xs[j] = (double)xi[j & 3];
-    } else if (NOISE == NOISE_EXTERNAL) {
-      xs[j] = ext[(long long)c * Nc + j];
+  for (int e = tid; e < CPT * Nc; e += blockDim.x) {
+    const int q = e / Nc, j = e % Nc, c = c0 + q;
+    double fv = 0.0, xv = 0.0;
+    if (c < K) {
+      fv = (double)f[c * fcs + pidx[j]];
+      if (NOISE == NOISE_PHILOX) {
+        float xi[4];
+        normals4(COARSE_SLOT + (uint32_t)(j >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+        xv = (double)xi[j & 3];
+      } else if (NOISE == NOISE_EXTERNAL) {
+        xv = ext[(long long)c * Nc + j];
+      }
     }
+    fs[e] = fv;
+    xs[e] = xv;
   }
   __syncthreads();
+  const int P = (int)blockDim.x >> 6;
   const int lane = tid & 63, part = tid >> 6;
   const int r = blockIdx.x * 64 + lane;
-  const int j0 = (Nc * part) / 4, j1 = (Nc * (part + 1)) / 4;
-  double s = 0.0;
+  const int j0 = (Nc * part) / P, j1 = (Nc * (part + 1)) / P;
+  double s[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) s[c] = 0.0;
   if (r < Nc) {
-    for (int j = j0; j < j1; ++j) s += Q[(long long)j * Nc + r] * fs[j];
+    for (int j = j0; j < j1; ++j) {
+      const double q = Q[(long long)j * Nc + r];
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) s[c] += q * fs[c * Nc + j];
+    }
     if (NOISE != NOISE_NONE)
-      for (int j = j0 > r ? j0 : r; j < j1; ++j) s += LT[(long long)j * Nc + r] * xs[j];
+      for (int j = j0 > r ? j0 : r; j < j1; ++j) {
+        const double l = LT[(long long)j * Nc + r];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) s[c] += l * xs[c * Nc + j];
+      }
   }
-  red[part][lane] = s;
+  __syncthreads();  // fs and xs are consumed: the same LDS holds the partial sums
+  double* red = sh;  // [P][CPT][64]
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) red[(part * CPT + c) * 64 + lane] = s[c];
   __syncthreads();
-  if (part == 0 && r < Nc) x[c * cs + pidx[r]] = (T)(red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
+  if (part == 0 && r < Nc) {
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      if (c0 + c >= K) break;
+      double v = 0.0;
+      for (int p = 0; p < P; ++p) v += red[(p * CPT + c) * 64 + lane];
+      x[(c0 + c) * cs + pidx[r]] = (T)v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -723,19 +723,34 @@ struct Engine : EngineBase {
     int l = C.nlevel - 1;
     Level& Lv = *C.L[l];
     xzero[l] = 0;  // k_coarse writes every interior value and reads no x
-    size_t sh = 2 * (size_t)C.Nc * sizeof(double);
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
-    dim3 grid((unsigned)((C.Nc + 63) / 64), C.K);
-    if (mode == NOISE_NONE)
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_NONE>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
-                                                          (int)C.Nc, xi_dev, nk);
-    else if (mode == NOISE_PHILOX)
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_PHILOX>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p, C.Qinv.p,
-                                                            (int)C.Nc, xi_dev, nk);
-    else
-      launch(grid, 256, sh, C.stream, k_coarse<T, NOISE_EXTERNAL>, X(l), f[l].p, Lv.g.cs, geo(l).fcs, C.cpidx.p, C.LinvT.p,
-                                                              C.Qinv.p, (int)C.Nc, xi_dev, nk);
+    // chains per block: as many as the LDS staging (2 CPT Nc doubles) allows
+    // within 64 KB, up to 8; more waves split the inner sum when fewer chains
+    // share a block
+    int cpt = 8;
+    while (cpt > 1 && (cpt > C.K || 2 * (size_t)cpt * C.Nc * sizeof(double) > 65536)) cpt >>= 1;
+    const int threads = cpt >= 8 ? 512 : 1024;
+    const size_t sh = std::max(2 * (size_t)cpt * C.Nc, (size_t)(threads / 64) * cpt * 64) * sizeof(double);
+    dim3 grid((unsigned)((C.Nc + 63) / 64), (unsigned)((C.K + cpt - 1) / cpt));
+    const long long fcs = geo(l).fcs;
+#define MGMC_COARSE(NOISE, CPT)                                                                                    \
+  launch(grid, threads, sh, C.stream, k_coarse<T, NOISE, CPT>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
+         C.Qinv.p, (int)C.Nc, C.K, xi_dev, nk)
+#define MGMC_COARSE_N(NOISE)                   \
+  if (cpt == 8) MGMC_COARSE(NOISE, 8);        \
+  else if (cpt == 4) MGMC_COARSE(NOISE, 4);   \
+  else if (cpt == 2) MGMC_COARSE(NOISE, 2);   \
+  else MGMC_COARSE(NOISE, 1);
+    if (mode == NOISE_NONE) {
+      MGMC_COARSE_N(NOISE_NONE)
+    } else if (mode == NOISE_PHILOX) {
+      MGMC_COARSE_N(NOISE_PHILOX)
+    } else {
+      MGMC_COARSE_N(NOISE_EXTERNAL)
+    }
+#undef MGMC_COARSE_N
+#undef MGMC_COARSE
   }
 
   void zero_x(int l) { fill(cur[l] ? x2[l].p : x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0); }
