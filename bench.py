@@ -117,7 +117,17 @@ def main():
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        # the same copy timed with HIP events: the streaming read + write rate
+        # this device reaches outside the sampler
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copy_ms": ms,
+                          "copy_GBps_read_plus_write": 8 * n / (ms * 1e-3) / 1e9}))
         return
 
     if args.gpus != 1:
