@@ -19,6 +19,10 @@ CASES = [
     (2, (16, 32), 3, 3),
     (3, (8, 8, 8), 0, 2),
     (3, (16, 16, 16), 3, 3),
+    # x extents that are not multiples of 8: the last 8-vertex group of a row
+    # holds the boundary vertex and padding
+    (2, (24, 20), 3, 2),
+    (3, (12, 16, 8), 3, 2),
 ]
 
 
