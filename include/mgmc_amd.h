@@ -209,6 +209,17 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
  * depend on these settings beyond rounding. */
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
+/* Autotuner (no reference counterpart): coordinate search over the tuning
+ * keys "rb_threads", "rb_blocks", "rr_threads" and "plane_sweep_stage3",
+ * which change launch shapes but no computed value; every candidate is timed
+ * over `ncycles` sampling cycles on the handle's stream and the fastest
+ * setting is kept. The chains' state and the sample counter are restored,
+ * so the sample path after tuning equals the untuned one bit for bit.
+ * best_ms (may be NULL) receives the best time per cycle. */
+int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms);
+/* Current value of one of the autotuned keys. */
+int mgmc_get_tuning(const mgmc_handle* h, const char* key, int* value);
+
 /* Profiling aid: with enable = 1 every level-0 sweep launch (one fused
  * red-black sweep, or one colour phase) is bracketed by HIP events on the
  * handle's stream (runs then launch eagerly).

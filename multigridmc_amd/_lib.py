@@ -85,6 +85,8 @@ _SIGS = {
     "mgmc_fem_stencil": [C.c_int, _vp, C.c_double, _vp],
     "mgmc_contract": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],
+    "mgmc_autotune": [_vp, C.c_int, C.POINTER(C.c_double)],
+    "mgmc_get_tuning": [_vp, C.c_char_p, C.POINTER(C.c_int)],
     "mgmc_profile": [_vp, C.c_int],
     "mgmc_profile_result": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }

@@ -1525,10 +1525,7 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out) {
   API_END
 }
 
-int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
-  API_BEGIN
-  if (!h || !key) throw Error(MGMC_ERR_ARG, "null argument");
-  std::string k(key);
+static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   if (k == "fused_sweep") h->C.tune_fused = value != 0;
   else if (k == "plane_sweep") h->C.tune_planes = value != 0;
   else if (k == "fold_prolongation") h->C.tune_fold_prolong = value != 0;
@@ -1561,6 +1558,107 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
   h->graph_valid = 0;
+}
+
+int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
+  API_BEGIN
+  if (!h || !key) throw Error(MGMC_ERR_ARG, "null argument");
+  apply_tuning(h, std::string(key), value);
+  API_END
+}
+
+static int get_tuning(const mgmc_handle* h, const std::string& k) {
+  if (k == "rb_threads") return h->C.tune_rb_threads;
+  if (k == "rb_blocks") return h->C.tune_rb_blocks;
+  if (k == "rr_threads") return h->C.tune_rr_threads;
+  if (k == "plane_sweep_stage3") return h->C.tune_planes_stage3;
+  throw Error(MGMC_ERR_ARG, "no autotuned key '" + k + "'");
+}
+
+int mgmc_get_tuning(const mgmc_handle* h, const char* key, int* value) {
+  API_BEGIN
+  if (!h || !key || !value) throw Error(MGMC_ERR_ARG, "null argument");
+  *value = get_tuning(h, std::string(key));
+  API_END
+}
+
+// Coordinate search over the launch-shape keys that leave every computed
+// value unchanged (tile heights, z-segment counts, workgroup sizes, where
+// the box sweeps read planes k +- 1 from): each candidate is timed over
+// `ncycles` eager sampling cycles with HIP events on the handle's stream
+// after one warm-up cycle, keys in a fixed order, the best value kept. The
+// chains' state and the sample counter are restored afterwards, so tuning
+// leaves the sample path unchanged.
+int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms) {
+  API_BEGIN
+  if (!h) throw Error(MGMC_ERR_ARG, "null handle");
+  if (ncycles < 1) throw Error(MGMC_ERR_ARG, "ncycles < 1");
+  Common& C = h->C;
+  sync(h);
+  const size_t esz = C.p.precision == MGMC_FP64 ? sizeof(double) : sizeof(float);
+  const size_t nbytes = (size_t)C.L[0]->g.cs * C.K * esz;
+  DevBuf<char> saved;
+  saved.alloc(nbytes);
+  HIP_CHECK(hipMemcpyAsync(saved.p, h->E->field(0, 0), nbytes, hipMemcpyDeviceToDevice, C.stream));
+  uint32_t ctr[2];
+  HIP_CHECK(hipMemcpyAsync(ctr, C.ctr.p, sizeof(ctr), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  const bool prof = C.prof;
+  C.prof = false;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  auto time_cycles = [&]() {
+    record_cycle(h, nullptr);
+    HIP_CHECK(hipEventRecord(e0, C.stream));
+    for (int s = 0; s < ncycles; ++s) record_cycle(h, nullptr);
+    HIP_CHECK(hipEventRecord(e1, C.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return (double)ms / ncycles;
+  };
+  const int max_threads = C.p.precision == MGMC_FP64 ? 512 : 1024;
+  struct Key {
+    const char* name;
+    std::vector<int> values;
+  };
+  std::vector<Key> keys = {{"rb_threads", {256, 384, 512, 768, 1024}},
+                           {"rb_blocks", {384, 512, 768, 1024, 1536, 2048}},
+                           {"rr_threads", {256, 512, 768, 1024}},
+                           {"plane_sweep_stage3", {1, 0}}};
+  double best = time_cycles();
+  try {
+    for (const Key& key : keys) {
+      const int start = get_tuning(h, key.name);
+      int keep = start;
+      for (int v : key.values) {
+        if (v == start) continue;
+        if ((std::string(key.name) == "rb_threads" || std::string(key.name) == "rr_threads") && v > max_threads)
+          continue;
+        apply_tuning(h, key.name, v);
+        const double t = time_cycles();
+        if (t < best) {
+          best = t;
+          keep = v;
+        }
+      }
+      apply_tuning(h, key.name, keep);
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    C.prof = prof;
+    throw;
+  }
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  C.prof = prof;
+  HIP_CHECK(hipMemcpyAsync(h->E->field(0, 0), saved.p, nbytes, hipMemcpyDeviceToDevice, C.stream));
+  HIP_CHECK(hipMemcpyAsync(C.ctr.p, ctr, sizeof(ctr), hipMemcpyHostToDevice, C.stream));
+  sync(h);
+  h->graph_valid = 0;
+  if (best_ms) *best_ms = best;
   API_END
 }
 

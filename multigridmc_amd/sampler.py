@@ -110,6 +110,18 @@ class _Handle:
     def set_tuning(self, key, value):
         check(_lib.lib().mgmc_set_tuning(self.h, key.encode(), int(value)))
 
+    def get_tuning(self, key):
+        v = C.c_int()
+        check(_lib.lib().mgmc_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, ncycles=5):
+        """Pick the fastest launch shapes for this handle (mgmc_autotune);
+        returns the best time per cycle in ms. The chain state is kept."""
+        ms = C.c_double()
+        check(_lib.lib().mgmc_autotune(self.h, int(ncycles), C.byref(ms)))
+        return ms.value
+
     def apply_operator(self, x):
         """y = A x + B Sigma^{-1} B^T x on the GPU for every row of x
         (LinearOperator::apply, src/linear_operator/linear_operator.hh:78-88)."""
