@@ -1024,8 +1024,15 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
   const int gx = t2 % ngx;
   const int j = 2 * (t2 / ngx) + (odd ? 1 : 2);
   const bool active = j <= g.n1 - 1;
-  const int pidx = blockIdx.x % nplanes;
-  const int chain = blockIdx.x / nplanes;
+  // workgroups are dispatched to the eight XCDs round-robin: deal out the
+  // (plane, chain) items so that an XCD runs consecutive planes of a chain,
+  // whose shared neighbour planes then meet in its L2 (FP32 cycle at 128^3
+  // x 64 chains: -1.2 % against the plain order)
+  int w = (int)blockIdx.x;
+  const int nb = (int)gridDim.x;
+  if ((nb & 7) == 0) w = (w & 7) * (nb >> 3) + (w >> 3);
+  const int pidx = w % nplanes;
+  const int chain = w / nplanes;
   const int k = kpar ? 2 * pidx + 1 : 2 * pidx + 2;
   const long long base = (long long)chain * g.cs + ((long long)k * g.py + j) * g.px + 8 * gx;
   const uint32_t sample = *nk.sample;
