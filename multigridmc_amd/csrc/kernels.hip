@@ -577,6 +577,19 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
 // stores, 32-bit index arithmetic only).
 // ---------------------------------------------------------------------------
 
+// Row-block grids (x: blocks of rows, y: chains). Workgroups are dispatched
+// to the eight XCDs round-robin in linear order (x fastest), so when the grid
+// divides by 8 the (block, chain) items are dealt out such that an XCD runs
+// consecutive row blocks of one chain, whose shared neighbour rows and
+// planes then meet in its L2.
+__device__ __forceinline__ void xcd_rowblock(int& bx, int& c) {
+  const int gxn = (int)gridDim.x, nb = (int)(gridDim.x * gridDim.y);
+  int lin = (int)blockIdx.x + (int)blockIdx.y * gxn;
+  if ((nb & 7) == 0) lin = (lin & 7) * (nb >> 3) + (lin >> 3);
+  bx = lin % gxn;
+  c = lin / gxn;
+}
+
 // r = f - A x on interior vertices (the sparse part of
 // src/linear_operator/linear_operator.hh:78-88, multigridmc_sampler.cc:125-126)
 template <typename T, int DIM, bool STAR>
@@ -584,9 +597,11 @@ __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const
                                                   DevGeom g, Stencil st) {
   const int ngx = g.px >> 3;
   const int nrows = (g.n1 - 1) * (DIM == 3 ? g.n2 - 1 : 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int bx, c;
+  xcd_rowblock(bx, c);
+  const int t = bx * blockDim.x + threadIdx.x;
   if (t >= ngx * nrows) return;
-  const int gx = t % ngx, row = t / ngx, c = blockIdx.y;
+  const int gx = t % ngx, row = t / ngx;
   const int j = row % (g.n1 - 1) + 1;
   const int k = DIM == 3 ? row / (g.n1 - 1) + 1 : 0;
   const long long pl = (long long)g.px * g.py;
@@ -649,9 +664,11 @@ template <typename T, int DIM>
 __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc) {
   const int nq = (gc.n0 - 1 + 4) / 4;  // groups of four coarse vertices covering I = 0 .. n0c - 1
   const int nrows = (gc.n1 - 1) * (DIM == 3 ? gc.n2 - 1 : 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int bx, c;
+  xcd_rowblock(bx, c);
+  const int t = bx * blockDim.x + threadIdx.x;
   if (t >= nq * nrows) return;
-  const int gq = t % nq, row = t / nq, c = blockIdx.y;
+  const int gq = t % nq, row = t / nq;
   const int J = row % (gc.n1 - 1) + 1;
   const int Kz = DIM == 3 ? row / (gc.n1 - 1) + 1 : 0;
   const long long pl = (long long)gf.px * gf.py;
@@ -687,9 +704,11 @@ __global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* 
                                                     T alpha) {
   const int ngx = gf.px >> 3;
   const int nrows = (gf.n1 - 1) * (DIM == 3 ? gf.n2 - 1 : 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int bx, c;
+  xcd_rowblock(bx, c);
+  const int t = bx * blockDim.x + threadIdx.x;
   if (t >= ngx * nrows) return;
-  const int gx = t % ngx, row = t / ngx, c = blockIdx.y;
+  const int gx = t % ngx, row = t / ngx;
   const int j = row % (gf.n1 - 1) + 1;
   const int k = DIM == 3 ? row / (gf.n1 - 1) + 1 : 0;
   // coarse rows/planes touching fine row j / plane k, with weights
