@@ -36,9 +36,14 @@ def test_mg_solve_fp64(dim, n, nmeas, nlevel, cycle):
     assert rho.max() < 0.3
 
 
-def test_mg_precondition_matches_oracle():
+@pytest.mark.parametrize("npre,npost,cycle", [(1, 1, 1), (0, 1, 1), (1, 0, 1), (2, 1, 2), (0, 2, 2)])
+def test_mg_precondition_matches_oracle(npre, npost, cycle):
+    """One MG cycle (MultigridPreconditioner::solve) against the oracle's,
+    including cycles whose coarse visits start with a residual (no
+    pre-smoothing), end without post-smoothing, or repeat (W-cycle)."""
     op, ora = make_problem(3, (16, 16, 16), 3)
-    p = params(3)
+    p = params(3, cycle=cycle)
+    p.npresmooth, p.npostsmooth = npre, npost
     pre = mg.MultigridPreconditioner(op, p)
     b = np.random.default_rng(5).standard_normal(op.lattice.Nvertex)
     got = pre.apply(b)
