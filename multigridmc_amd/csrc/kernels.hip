@@ -855,6 +855,90 @@ __global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __r
   }
 }
 
+// The same coarsest-level products on the FP64 matrix cores. A workgroup
+// owns one 16 x 16 output tile (16 rows r0 .. r0 + 15 x 16 chains c0 ..),
+// its waves split the inner dimension into equal runs of k-steps of four
+// and reduce their partial tiles through LDS in a fixed order; each k-step
+// is one v_mfma_f64_16x16x4f64 (lane l holds A[l % 16][l / 16] of the
+// 16 x 4 operand and B[l / 16][l % 16] of the 4 x 16 one; item v of the
+// accumulator is D[l / 16 + 4 v][l % 16]). The workgroup stages the
+// right-hand sides and normals of its 16 chains in LDS (zero beyond K and
+// beyond Nc); L^{-T} is upper triangular, so its steps below row r0 are
+// skipped. LDS: 2 * 16 * round_up(Nc, 4) + 256 * waves doubles.
+#ifndef MGMC_EMULATION
+typedef double mgmc_d4 __attribute__((ext_vector_type(4)));
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T* __restrict__ f, long long cs,
+                                                     long long fcs, const long long* __restrict__ pidx,
+                                                     const double* __restrict__ LT, const double* __restrict__ Q,
+                                                     int Nc, int K, const double* __restrict__ ext, NoiseKey nk) {
+  MGMC_DYNAMIC_SHARED(double, sh);
+  const int Nc4 = (Nc + 3) & ~3;
+  double* fs = sh;                  // [16][Nc4]
+  double* xs = sh + 16 * Nc4;       // [16][Nc4]
+  double* red = sh + 32 * Nc4;      // [waves][256]
+  const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+  const int tid = threadIdx.x, nw = (int)blockDim.x >> 6, w = tid >> 6, lane = tid & 63;
+  for (int e = tid; e < 16 * Nc4; e += blockDim.x) {
+    const int q = e / Nc4, j = e % Nc4, c = c0 + q;
+    double fv = 0.0, xv = 0.0;
+    if (c < K && j < Nc) {
+      fv = (double)f[c * fcs + pidx[j]];
+      if (NOISE == NOISE_PHILOX) {
+        float xi[4];
+        normals4(COARSE_SLOT + (uint32_t)(j >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+        xv = (double)xi[j & 3];
+      } else if (NOISE == NOISE_EXTERNAL) {
+        xv = ext[(long long)c * Nc + j];
+      }
+    }
+    fs[e] = fv;
+    xs[e] = xv;
+  }
+  __syncthreads();
+  const int ra = r0 + (lane & 15), ka = lane >> 4, cb = lane & 15;
+  // this wave's k-steps: [s0, s1) of the Nc4 / 4 steps; four accumulators
+  // (four matrix-core chains in flight), operand loads of a trip issued
+  // together
+  const int nstep = Nc4 >> 2;
+  const int per = (nstep + nw - 1) / nw;
+  const int s0 = w * per, s1 = s0 + per < nstep ? s0 + per : nstep;
+  mgmc_d4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = mgmc_d4{0.0, 0.0, 0.0, 0.0};
+  auto steps = [&](const double* __restrict__ M, const double* B, int sbeg, bool upper) {
+    for (int st = sbeg; st < s1; st += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jj = 4 * (st + u) + ka;
+        const bool in = st + u < s1;
+        const bool ok = in && ra < Nc && jj < Nc && (!upper || jj >= ra);
+        a[u] = ok ? M[(long long)jj * Nc + ra] : 0.0;
+        b[u] = in ? B[cb * Nc4 + jj] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc[u], 0, 0, 0);
+    }
+  };
+  steps(Q, fs, s0, false);
+  if (NOISE != NOISE_NONE) steps(LT, xs, s0 > (r0 >> 2) ? s0 : (r0 >> 2), true);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[w * 256 + v * 64 + lane] = ((acc[0][v] + acc[1][v]) + acc[2][v]) + acc[3][v];
+  __syncthreads();
+  if (w == 0) {
+    const int c = c0 + cb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double t = 0.0;
+      for (int q = 0; q < nw; ++q) t += red[q * 256 + v * 64 + lane];
+      const int r = r0 + ka + 4 * v;
+      if (r < Nc && c < K) x[c * cs + pidx[r]] = (T)t;
+    }
+  }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // One whole red-black SOR/Gibbs sweep of a 3D star-stencil level in a single
 // pass over HBM. A workgroup owns full x-rows (one thread per aligned group

@@ -130,6 +130,7 @@ struct Common {
   int tune_rb_lds_kb = 64;    // LDS budget of its tiles (KB; the device allows up to max_lds)
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
+  int tune_coarse_mfma = 1;   // coarsest-level products on the FP64 matrix cores (8+ chains)
   int tune_rr_threads = 512;
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
   int tune_march_blocks = 512; // ... workgroups they aim for
@@ -391,6 +392,9 @@ struct Engine : EngineBase {
         reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_NONE, false>),
         reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_PHILOX, false>),
         reinterpret_cast<const void*>(k_sweep_box_planes<T, NOISE_EXTERNAL, false>),
+        reinterpret_cast<const void*>(k_coarse_mfma<T, NOISE_NONE>),
+        reinterpret_cast<const void*>(k_coarse_mfma<T, NOISE_PHILOX>),
+        reinterpret_cast<const void*>(k_coarse_mfma<T, NOISE_EXTERNAL>),
     };
     for (const void* k : ks) HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, C.max_lds));
 #endif
@@ -725,6 +729,25 @@ struct Engine : EngineBase {
     xzero[l] = 0;  // k_coarse writes every interior value and reads no x
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
+    const long long fcs = geo(l).fcs;
+#ifndef MGMC_EMULATION
+    // FP64 matrix cores (k_coarse_mfma) for batches of 8 or more chains when
+    // 16 chains' right-hand sides and normals fit in LDS; with fewer chains a
+    // 16-chain tile is mostly empty and the vector kernel below is faster
+    // (one chain at 343 unknowns: 17 against 29 us)
+    const size_t sh_mfma = (2 * 16 * (size_t)((C.Nc + 3) & ~3LL) + 8 * 256) * sizeof(double);
+    if (C.tune_coarse_mfma && C.K >= 8 && sh_mfma <= (size_t)C.max_lds) {
+      dim3 grid((unsigned)((C.Nc + 15) / 16), (unsigned)((C.K + 15) / 16));
+#define MGMC_COARSE_MFMA(NOISE)                                                                                    \
+  launch(grid, 512, sh_mfma, C.stream, k_coarse_mfma<T, NOISE>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
+         C.Qinv.p, (int)C.Nc, C.K, xi_dev, nk)
+      if (mode == NOISE_NONE) MGMC_COARSE_MFMA(NOISE_NONE);
+      else if (mode == NOISE_PHILOX) MGMC_COARSE_MFMA(NOISE_PHILOX);
+      else MGMC_COARSE_MFMA(NOISE_EXTERNAL);
+#undef MGMC_COARSE_MFMA
+      return;
+    }
+#endif
     // chains per block: as many as the LDS staging (2 CPT Nc doubles) allows
     // within 64 KB, up to 8; more waves split the inner sum when fewer chains
     // share a block
@@ -733,7 +756,6 @@ struct Engine : EngineBase {
     const int threads = cpt >= 8 ? 512 : 1024;
     const size_t sh = std::max(2 * (size_t)cpt * C.Nc, (size_t)(threads / 64) * cpt * 64) * sizeof(double);
     dim3 grid((unsigned)((C.Nc + 63) / 64), (unsigned)((C.K + cpt - 1) / cpt));
-    const long long fcs = geo(l).fcs;
 #define MGMC_COARSE(NOISE, CPT)                                                                                    \
   launch(grid, threads, sh, C.stream, k_coarse<T, NOISE, CPT>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
          C.Qinv.p, (int)C.Nc, C.K, xi_dev, nk)
@@ -1531,6 +1553,7 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   else if (k == "fold_prolongation") h->C.tune_fold_prolong = value != 0;
   else if (k == "fused_residual") h->C.tune_fused_rr = value != 0;
   else if (k == "box_march") h->C.tune_march = value != 0;
+  else if (k == "coarse_mfma") h->C.tune_coarse_mfma = value != 0;
   else if (k == "rb_lds_kb") {
     if (value < 16 || value > 160) throw Error(MGMC_ERR_ARG, "rb_lds_kb must lie in [16, 160]");
     h->C.tune_rb_lds_kb = value;

@@ -123,6 +123,27 @@ def test_coarse_parity(case):
         assert _rel(got_d[c], omg.coarse_solve(f[c])) < 1e-11
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+@pytest.mark.parametrize("n,nlevel,K", [((16, 16, 24), 2, 40), ((64, 64, 64), 4, 17), ((24, 20), 2, 9)])
+def test_coarse_mfma_matches_vector_kernel(precision, tol, n, nlevel, K):
+    """The coarsest-level products on the FP64 matrix cores equal the vector
+    kernel's for sample (injected xi and Philox normals) and solve, with
+    partial 16-row and 16-chain tiles (539, 343 and 99 coarse unknowns; K not
+    a multiple of 16; the matrix-core kernel serves 8 or more chains)."""
+    op, ora = make_problem(len(n), n, 3)
+    p = params(nlevel)
+    a = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    b = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=9)
+    b.set_tuning("coarse_mfma", 0)
+    Nc = a.ndof(nlevel - 1)
+    rng = np.random.default_rng(13)
+    f, xi = rng.standard_normal((K, Nc)), rng.standard_normal((K, Nc))
+    for mode, x in ((2, xi), (0, None), (1, None)):
+        ga, gb = a.coarse(f, xi_mode=mode, xi=x), b.coarse(f, xi_mode=mode, xi=x)
+        assert np.abs(ga - gb).max() <= tol * np.abs(gb).max(), mode
+
+
 def test_apply_operator():
     op, ora = make_problem(3, (16, 16, 16), 3)
     s = mg.MultigridMCSampler(op, params(3), nchains=2)
