@@ -130,7 +130,7 @@ struct Common {
   int tune_rb_lds_kb = 64;    // LDS budget of its tiles (KB; the device allows up to max_lds)
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
-  int tune_coarse_mfma = 1;   // coarsest-level products on the FP64 matrix cores (8+ chains)
+  int tune_coarse_mfma = 1;   // coarsest-level products on the FP64 matrix cores
   int tune_rr_threads = 512;
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
   int tune_march_blocks = 512; // ... workgroups they aim for
@@ -731,12 +731,12 @@ struct Engine : EngineBase {
     NoiseKey nk = key(tag);
     const long long fcs = geo(l).fcs;
 #ifndef MGMC_EMULATION
-    // FP64 matrix cores (k_coarse_mfma) for batches of 8 or more chains when
-    // 16 chains' right-hand sides and normals fit in LDS; with fewer chains a
-    // 16-chain tile is mostly empty and the vector kernel below is faster
-    // (one chain at 343 unknowns: 17 against 29 us)
+    // FP64 matrix cores (k_coarse_mfma) when 16 chains' right-hand sides and
+    // normals fit in LDS (at 343 unknowns: 64 chains 15.7 instead of 44.5 us,
+    // one chain faster than the vector kernel too); the vector kernel below
+    // serves larger coarse levels
     const size_t sh_mfma = (2 * 16 * (size_t)((C.Nc + 3) & ~3LL) + 8 * 256) * sizeof(double);
-    if (C.tune_coarse_mfma && C.K >= 8 && sh_mfma <= (size_t)C.max_lds) {
+    if (C.tune_coarse_mfma && sh_mfma <= (size_t)C.max_lds) {
       dim3 grid((unsigned)((C.Nc + 15) / 16), (unsigned)((C.K + 15) / 16));
 #define MGMC_COARSE_MFMA(NOISE)                                                                                    \
   launch(grid, 512, sh_mfma, C.stream, k_coarse_mfma<T, NOISE>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
