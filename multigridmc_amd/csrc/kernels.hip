@@ -383,15 +383,19 @@ constexpr int LR_CPB = 1;
 template <typename T>
 __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
                                                 const long long* __restrict__ bidx, const double* __restrict__ bval,
-                                                int m, int K, double* __restrict__ z) {
-  const int kcol = blockIdx.x, c0 = blockIdx.y * LR_CPB;
+                                                int m, int K, double* __restrict__ z, int nsplit) {
+  // block (kcol * nsplit + part, chain group): part `part` of nsplit equal
+  // runs of the column's entries; the partial sums are added in order by
+  // the solve
+  const int kcol = blockIdx.x / nsplit, part = blockIdx.x % nsplit, c0 = blockIdx.y * LR_CPB;
   const int nc = K - c0 < LR_CPB ? K - c0 : LR_CPB;
   double s[LR_CPB];
 #pragma unroll
   for (int q = 0; q < LR_CPB; ++q) s[q] = 0.0;
-  const int e1 = bptr[kcol + 1];
+  const int cb0 = bptr[kcol], clen = bptr[kcol + 1] - cb0;
+  const int e1 = cb0 + (int)(((long long)clen * (part + 1)) / nsplit);
   const int stride = blockDim.x;
-  int e = bptr[kcol] + threadIdx.x;
+  int e = cb0 + (int)(((long long)clen * part) / nsplit) + threadIdx.x;
   // four entries per trip: their index and value loads, then their x loads,
   // are issued together (latency-bound gather)
   for (; e + 3 * stride < e1; e += 4 * stride) {
@@ -430,7 +434,7 @@ __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long lo
   __syncthreads();
   if (threadIdx.x < nc) {
     const int q = threadIdx.x;
-    z[(long long)(c0 + q) * m + kcol] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+    z[((long long)(c0 + q) * m + kcol) * nsplit + part] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
   }
 }
 
@@ -439,11 +443,13 @@ template <int NOISE>
 __global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z, const double* __restrict__ M,
                                                    const double* __restrict__ sqrt_sigma, double zc,
                                                    const double* __restrict__ ext_lr, double* __restrict__ u, int m,
-                                                   NoiseKey nk) {
+                                                   NoiseKey nk, int nsplit) {
   __shared__ double a[MMAX];
   const int c = blockIdx.x, tid = threadIdx.x;
   if (tid < m) {
-    double v = zc * z[(long long)c * m + tid];
+    double zs = 0.0;
+    for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + tid) * nsplit + p];
+    double v = zc * zs;
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
       normals4(LOWRANK_SLOT + (uint32_t)(tid >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
@@ -516,7 +522,8 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
                                                          const double* __restrict__ val, int nrows,
                                                          const double* __restrict__ z, const double* __restrict__ M,
                                                          const double* __restrict__ sqrt_sigma, double zc,
-                                                         const double* __restrict__ ext_lr, int m, int K, NoiseKey nk) {
+                                                         const double* __restrict__ ext_lr, int m, int K, NoiseKey nk,
+                                                         int nsplit) {
   __shared__ double a[LR_CPU][MMAX];
   __shared__ double us[LR_CPU][MMAX];
   const int c0 = blockIdx.y * LR_CPU;
@@ -536,7 +543,9 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
   for (int q = 0; q < LR_CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
   for (int e = tid; e < nc * m; e += blockDim.x) {
     const int q = e / m, i = e % m, c = c0 + q;
-    double v = zc * z[(long long)c * m + i];
+    double zs = 0.0;
+    for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + i) * nsplit + p];
+    double v = zc * zs;
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
       normals4(LOWRANK_SLOT + (uint32_t)(i >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);

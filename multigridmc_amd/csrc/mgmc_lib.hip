@@ -94,12 +94,18 @@ struct Level {
   int64_t N;
   // low rank
   DevBuf<int32_t> bptr;
+  int nsplit = 1;  // blocks per column of B in k_lr_dot
   DevBuf<long long> bidx;
   DevBuf<double> bval;
   DevRows Brows;
   DevRows W[2];
   DevBuf<double> Sinv[2];
 };
+
+// k_lr_dot: B columns are split into up to kMaxLrSplit blocks of about
+// kLrSplitEntries entries (one batched gather trip of 256 threads each)
+constexpr int kMaxLrSplit = 8;
+constexpr long long kLrSplitEntries = 512;
 
 struct Common {
   int K = 1;
@@ -538,22 +544,26 @@ struct Engine : EngineBase {
     int noise = NOISE_NONE;
     const double* extlr = nullptr;
     uint32_t tag = 0;
+    int nsplit = 1;  // partial sums per entry of z
   } lr_next;
   bool lr_fused() const { return C.K <= LR_CPU; }
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
-    dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
-    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p);
-    lr_next = LrSolve{M, noise, extlr, tag};
+    dim3 grid(C.m * Lv.nsplit, (C.K + LR_CPB - 1) / LR_CPB);
+    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p,
+           Lv.nsplit);
+    lr_next = LrSolve{M, noise, extlr, tag, Lv.nsplit};
     if (lr_fused()) return;
     NoiseKey nk = key(tag);
     if (noise == NOISE_NONE)
-      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+             Lv.nsplit);
     else if (noise == NOISE_PHILOX)
-      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk);
+      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+             Lv.nsplit);
     else
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
-             nk);
+             nk, Lv.nsplit);
   }
 
   // x[rows(R)] += R u with u from the last lr_dot_solve
@@ -568,7 +578,7 @@ struct Engine : EngineBase {
     NoiseKey nk = key(lr_next.tag);
 #define MGMC_LRSU(NOISE)                                                                                        \
   launch(grid, 256, 0, C.stream, k_lr_solve_update<T, NOISE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p,      \
-         R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk)
+         R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk, lr_next.nsplit)
     if (lr_next.noise == NOISE_NONE) MGMC_LRSU(NOISE_NONE);
     else if (lr_next.noise == NOISE_PHILOX) MGMC_LRSU(NOISE_PHILOX);
     else MGMC_LRSU(NOISE_EXTERNAL);
@@ -916,9 +926,11 @@ struct Engine : EngineBase {
     if (C.m > 0) {
       dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
       NoiseKey nk = key(0);
-      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p);
+      grid.x *= F.nsplit;
+      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p,
+             F.nsplit);
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, C.sigma_inv_mat.p, C.sqrt_sigma.p, -1.0, nullptr,
-             C.u.p, C.m, nk);
+             C.u.p, C.m, nk, F.nsplit);
       dim3 g2(nblocks(F.Brows.nrows), (C.K + LR_CPU - 1) / LR_CPU);
       if (F.Brows.nrows)
         launch(g2, 256, 0, C.stream, k_lr_update<double>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
@@ -1045,6 +1057,11 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     Lv->N = HL.g.N;
     if (C.m > 0) {
       Lv->bptr.upload(HL.bptr.data(), HL.bptr.size());
+      {
+        const long long nnz = HL.bptr.empty() ? 0 : HL.bptr.back();
+        const long long per_col = C.m > 0 ? nnz / C.m : 0;
+        Lv->nsplit = (int)std::max(1LL, std::min((long long)kMaxLrSplit, per_col / kLrSplitEntries));
+      }
       std::vector<long long> bi(HL.bidx.begin(), HL.bidx.end());
       Lv->bidx.upload(bi.data(), bi.size());
       Lv->bval.upload(HL.bval.data(), HL.bval.size());
@@ -1064,7 +1081,7 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     }
     C.sqrt_sigma.upload(ss.data(), ss.size());
     C.sigma_inv_mat.upload(sim.data(), sim.size());
-    C.z.alloc((size_t)C.m * C.K);
+    C.z.alloc((size_t)C.m * C.K * kMaxLrSplit);
     C.u.alloc((size_t)C.m * C.K);
   }
   C.Nc = C.H.coarse.Nc;

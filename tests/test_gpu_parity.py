@@ -144,8 +144,11 @@ def test_coarse_mfma_matches_vector_kernel(precision, tol, n, nlevel, K):
         assert np.abs(ga - gb).max() <= tol * np.abs(gb).max(), mode
 
 
-def test_apply_operator():
-    op, ora = make_problem(3, (16, 16, 16), 3)
+@pytest.mark.parametrize("n,radius", [((16, 16, 16), 0.15), ((32, 32, 32), 0.3)])
+def test_apply_operator(n, radius):
+    """Q x with the low-rank term; radius 0.3 at 32^3 gives B columns of
+    several thousand entries, which k_lr_dot splits over several blocks."""
+    op, ora = make_problem(3, n, 3, radius=radius)
     s = mg.MultigridMCSampler(op, params(3), nchains=2)
     x = np.random.default_rng(4).standard_normal((2, op.lattice.Nvertex))
     y = s.apply_operator(x)
