@@ -75,8 +75,9 @@ def _posterior_moments(ora, f, probes):
 @pytest.mark.parametrize("smoother,cycle", [("SSOR", 1), ("SOR", 2)])
 def test_mgmc_posterior_moments(precision, smoother, cycle):
     """Batched MGMC chains sample N(Q^-1 f, Q^-1): means and covariances at
-    probe vertices agree with the exact posterior within 4 Monte Carlo
-    standard errors (the statistical test of test_sampler.hh:113-153)."""
+    probe vertices agree with the exact posterior within 3 Monte Carlo
+    standard errors (the statistical test of test_sampler.hh:113-153; the
+    largest deviations with these seeds are 2.6 and 2.8 errors)."""
     dim, n = 2, (16, 16)
     op, ora = make_problem(dim, n, 2, radius=0.15, var=1e-3)
     N = op.lattice.Nvertex
@@ -97,11 +98,11 @@ def test_mgmc_posterior_moments(precision, smoother, cycle):
     neff = K * ns / max(tau, 1.0)
     m_emp = obs.mean(axis=(0, 1))
     se_mean = np.sqrt(np.diag(cov) / neff)
-    assert np.all(np.abs(m_emp - mu[probes]) < 4 * se_mean + 1e-6)
+    assert np.all(np.abs(m_emp - mu[probes]) < 3 * se_mean + 1e-6)
     d = obs.reshape(-1, len(probes)) - m_emp
     cov_emp = d.T @ d / d.shape[0]
     se_cov = np.sqrt((np.outer(np.diag(cov), np.diag(cov)) + cov**2) / neff)
-    assert np.all(np.abs(cov_emp - cov) < 4 * se_cov + 1e-6)
+    assert np.all(np.abs(cov_emp - cov) < 3 * se_cov + 1e-6)
     assert tau < 3.0
 
 
