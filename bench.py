@@ -7,9 +7,14 @@ smoothing (omega = 1), 5 levels (coarsest 7^3 = 343 unknowns, Cholesky).
 A step is one MGMC cycle over the whole batch of chains.
 
 value = IACT-adjusted independent samples per second of the batched FP32 form
-(64 chains): K * steps / (t_wall * tau_int), tau_int the largest Sokal
-estimate over the probe vertices. The plain single-chain FP64 form is
-measured in the same run and reported beside it, with the ratio.
+(64 chains): K * steps / (t_wall * tau_int). t_wall covers exactly `steps`
+replays of the captured cycle (observables recorded into device memory; the
+buffer and the graph exist before the clock starts). tau_int is the largest
+Sokal estimate over the probe vertices, taken from a fixed run of --tau-steps
+cycles per chain after the timed one, so that it does not depend on --steps.
+The plain single-chain FP64 form runs --fp64-steps cycles (timed, and its
+IACT sample) in the same job and is reported beside it, with the ratio, and
+both forms' probe covariances are compared with the exact posterior.
 """
 import argparse
 import json
@@ -37,12 +42,38 @@ def problem(n, m, seed):
 
 
 def tau_max(obs):
+    """Largest and median Sokal tau_int over the probes of obs[sample, chain, probe]."""
     import multigridmc_amd as mg
     taus = [mg.tau_int_chains(obs[:, :, p])[0] for p in range(obs.shape[2])]
     return float(max(taus)), float(np.median(taus))
 
 
-def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed, use_graph=True, tuning=None):
+def covariance_error(obs, mean_exact, cov_exact, tau):
+    """Error of the probe statistics of obs[sample, chain, probe] against the
+    exact posterior: relative Frobenius error of the probe covariance, its
+    Monte Carlo noise floor (for Gaussian samples Var C_ij = (C_ii C_jj +
+    C_ij^2) / n_eff, n_eff = samples / tau), the RMS and largest relative
+    variance errors beside their noise, and the largest mean error in
+    posterior standard deviations."""
+    z = obs.reshape(-1, obs.shape[2])
+    n = z.shape[0]
+    mu = z.mean(axis=0)
+    d = z - mu
+    cov = d.T @ d / n
+    norm = np.linalg.norm(cov_exact)
+    err = float(np.linalg.norm(cov - cov_exact) / norm)
+    dg = np.diag(cov_exact)
+    floor = float(np.sqrt(np.sum(np.outer(dg, dg) + cov_exact ** 2) * tau / n) / norm)
+    rel_var = (np.diag(cov) - dg) / dg
+    return {"rel_frobenius": err, "mc_noise_floor": floor, "samples_per_probe": n,
+            # relative variance errors: Monte Carlo noise sqrt(2 tau / n) each
+            "rms_rel_variance_error": float(np.sqrt(np.mean(rel_var ** 2))),
+            "rel_variance_mc_noise": float(np.sqrt(2.0 * tau / n)),
+            "max_rel_variance_error": float(np.max(np.abs(rel_var))),
+            "max_mean_error_in_sd": float(np.max(np.abs(mu - mean_exact) / np.sqrt(dg)))}
+
+
+def new_sampler(op, f, params, nchains, precision, probes, seed, tuning=None):
     import multigridmc_amd as mg
     s = mg.MultigridMCSampler(op, params, nchains=nchains, precision=precision, seed=seed)
     for k, v in (tuning or {}).items():
@@ -50,11 +81,35 @@ def run_form(op, f, params, nchains, precision, warmup, steps, probes, seed, use
     s.fix_rhs(f)
     s.set_state(np.zeros((nchains, op.lattice.Nvertex)))
     s.set_probes(probes)
-    s.run(warmup, record=False, use_graph=use_graph)
+    return s
+
+
+def timed_run(s, warmup, steps, rows, use_graph):
+    """Warm-up and `steps` timed cycles, both recording the probe observables
+    into one device buffer of `rows` rows, so that the buffer exists and the
+    cycle graph is captured before the clock starts; the timed region holds
+    graph replays only, bracketed by device synchronisations."""
+    import torch
+    obs = torch.empty((rows, s.nchains, s.nprobe), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    s.run_dev(max(warmup, 1), obs.data_ptr(), use_graph)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    obs = s.run(steps, use_graph=use_graph)  # synchronises the handle's stream before returning
+    s.run_dev(steps, obs.data_ptr(), use_graph)
+    torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    return s, obs, t
+    return t, obs[:steps].cpu().numpy()
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(n, m, seed, budget_s, nthreads):
@@ -97,7 +152,12 @@ def main():
     ap.add_argument("--chains", type=int, default=64)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--m", type=int, default=32)
-    ap.add_argument("--fp64-steps", type=int, default=2000)
+    ap.add_argument("--fp64-steps", type=int, default=20000,
+                    help="cycles of the single-chain FP64 run (timed; also its IACT sample)")
+    ap.add_argument("--tau-steps", type=int, default=400, help="cycles per chain of the FP32 IACT estimate")
+    ap.add_argument("--stat-steps", type=int, default=2000,
+                    help="cycles per chain of the FP32 statistics run (covariance error; >= --tau-steps)")
+    ap.add_argument("--no-exact", action="store_true", help="skip the exact probe moments (FP64 MG solves)")
     ap.add_argument("--probes", type=int, default=32)
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
@@ -135,7 +195,9 @@ def main():
               f"GPUs with no per-step communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    import torch  # noqa: F401  (before the library: one HIP runtime in the process)
     import multigridmc_amd as mg
+    from multigridmc_amd import measurements
 
     n, K = args.n, args.chains
     op, f, centres, y, var = problem(n, args.m, args.seed)
@@ -144,12 +206,18 @@ def main():
     probes = np.sort(rng.choice(N, size=args.probes, replace=False))
     params = mg.MultigridParameters(nlevel=5, smoother="SSOR", npresmooth=1, npostsmooth=1, omega=1.0, cycle=1,
                                     coarse_solver="Cholesky")
-
-    # batched FP32 (the metric's form)
     g = not args.no_graph
     tuning = {"fused_sweep": 0} if args.colour_phases else {}
-    s32, obs32, t32 = run_form(op, f, params, K, "fp32", args.warmup, args.steps, probes, 1, g, tuning)
-    tau32, tau32_med = tau_max(obs32)
+
+    # batched FP32 (the metric's form): exactly --steps timed cycles
+    s32 = new_sampler(op, f, params, K, "fp32", probes, 1, tuning)
+    t32, _ = timed_run(s32, args.warmup, args.steps, max(args.warmup, args.steps, 1), g)
+
+    # IACT and covariance on a fixed run that does not depend on --steps: the
+    # chains continue from the timed run (in equilibrium after the warm-up)
+    obs32 = s32.run(args.stat_steps, use_graph=g)
+    tau32, tau32_med = tau_max(obs32[:args.tau_steps])
+    tau32_long, _ = tau_max(obs32)
     ess32 = K * args.steps / (t32 * tau32)
 
     # dominant kernel: the level-0 sweep, timed with HIP events on the handle's stream
@@ -157,19 +225,21 @@ def main():
     s32.run(10, record=False)
     ms_sweep, nlaunch = s32.profile_result()
     s32.profile(0)
-    # algorithmic bytes: interior vertices only; f is one vector shared by the
-    # chains (read once per launch)
     w = 4
+    Nc = (n // 2 - 1) ** 3
     if args.colour_phases:
         kernel = "k_sweep<float,3,star,philox> (level-0 red-black colour phase)"
-        bytes_per_launch = w * N * K + w * N * K // 2 + w * N  # read x (all), write x (one colour), f
+        # one colour: read x (all, for the neighbours), f and write x of half the vertices
+        bytes_per_launch = w * N * K + w * N * K // 2 + w * N * K // 2
+        bytes_shared_f = w * N * K + w * N * K // 2 + w * N
     else:
         kernel = "k_sweep_rb3<float,philox> (level-0 red-black sweep, one pass)"
-        # read x, write x, f once; the first post-smoothing launch of the cycle
-        # also reads the level-1 correction (one of the 2 (npre + npost) launches)
-        Nc = (n // 2 - 1) ** 3
+        # per directional sweep and chain 3 w N_0 (read x, read f, write x); the
+        # first post-smoothing launch of the cycle (one of 2 (npre + npost)) also
+        # reads the level-1 correction it adds while loading
         nlaunch_cycle = 2 * (params.npresmooth + params.npostsmooth)
-        bytes_per_launch = 2 * w * N * K + w * N + w * Nc * K / nlaunch_cycle
+        bytes_per_launch = 3 * w * N * K + w * Nc * K / nlaunch_cycle
+        bytes_shared_f = 2 * w * N * K + w * N + w * Nc * K / nlaunch_cycle
     achieved = bytes_per_launch / (ms_sweep * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", ("sweep_l0_fp32_128cube_k64_traffic.json" if args.colour_phases
@@ -178,10 +248,18 @@ def main():
         with open(tf) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    # plain single-chain FP64
-    s64, obs64, t64 = run_form(op, f, params, 1, "fp64", args.warmup, args.fp64_steps, probes, 2, g, tuning)
+    # plain single-chain FP64: its timed run is the fixed-length IACT run
+    s64 = new_sampler(op, f, params, 1, "fp64", probes, 2, tuning)
+    t64, obs64 = timed_run(s64, args.warmup, args.fp64_steps, max(args.warmup, args.fp64_steps), g)
     tau64, tau64_med = tau_max(obs64)
     ess64 = args.fp64_steps / (t64 * tau64)
+
+    # exact probe moments from FP64 multigrid solves (the posterior mean and
+    # b_p^T Q^{-1} b_q), against which both forms' samples are checked
+    cov_exact = None
+    if not args.no_exact:
+        obsv = [(np.array([p]), np.array([1.0])) for p in probes]
+        mean_exact, cov_exact = measurements.observed_moments(op, np.zeros(N), y, obsv)
 
     out = {
         "metric": METRIC,
@@ -205,8 +283,12 @@ def main():
             "nlevel": 5,
             "probes": args.probes,
         },
-        "tau_int": {"fp32_batched_max": tau32, "fp32_batched_median": tau32_med, "fp64_single_max": tau64,
-                    "fp64_single_median": tau64_med, "estimator": "Sokal automatic window c=5"},
+        "tau_int": {"fp32_batched_max": tau32, "fp32_batched_median": tau32_med,
+                    "fp32_samples_per_probe": K * args.tau_steps,
+                    "fp32_batched_max_long_run": tau32_long, "fp32_long_run_samples_per_probe": K * args.stat_steps,
+                    "fp64_single_max": tau64, "fp64_single_median": tau64_med,
+                    "fp64_samples_per_probe": args.fp64_steps,
+                    "estimator": "Sokal automatic window c=5, chain-averaged autocovariance about the pooled mean"},
         "fp64_single_chain": {"value": ess64, "unit": "indep. samples/s", "steps": args.fp64_steps,
                               "ms_per_step": 1e3 * t64 / args.fp64_steps},
         "speedup_vs_fp64_single": ess32 / ess64,
@@ -214,13 +296,19 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel,
                      "avg_launch_ms": ms_sweep, "launches_timed": nlaunch,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "achieved_f_read_once": bytes_shared_f / (ms_sweep * 1e-3) / 1e9},
         "cpu_baseline": None,
     }
+    if cov_exact is not None:
+        out["covariance_error_vs_fp64_exact"] = {
+            "fp32_batched": covariance_error(obs32, mean_exact, cov_exact, tau32_long),
+            "fp64_single": covariance_error(obs64, mean_exact, cov_exact, tau64)}
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         try:
             out["cpu_baseline"] = cpu_baseline(n, args.m, args.seed, args.cpu_budget, nthreads)
+            out["cpu_baseline"]["cpu_model"] = host_cpu_model()
         except Exception as e:  # the GPU line must still print
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
