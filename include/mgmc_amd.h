@@ -191,6 +191,18 @@ int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil);
  * scale). lam, mu: nchains x N. */
 int mgmc_contract(mgmc_handle* h, const double* stencil, const double* lam, const double* mu, double* out);
 
+/* mgmc_contract without allocation. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). lam_dev, mu_dev:
+ * device arrays of nchains x N FP64 in reference vertex order; out_dev:
+ * device array of 2 x nchains receiving what mgmc_contract writes to `out`.
+ * The partial sums use a fixed device array of 65,536 doubles (512 KiB per
+ * GPU, held until the process ends); nchains x 2 x min(256, ceil(N / 2048))
+ * must fit in it, else MGMC_ERR_ARG. Work is queued on the handle's stream
+ * without synchronising; calls that overlap on one GPU race for the array. */
+int mgmc_contract_noalloc(mgmc_handle* h, const double* stencil, const double* lam_dev, const double* mu_dev,
+                          double* out_dev);
+
 /* Export the level's stencil as used by the kernels (27 or 9 entries, x fastest,
  * offsets -1..1 per direction) and the colour count (2 = red-black). */
 int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* ncolours);

@@ -84,6 +84,7 @@ _SIGS = {
     "mgmc_measurement_vector": [C.c_int, _vp, _vp, C.c_double, C.c_int64, _vp, _vp, C.POINTER(C.c_int64)],
     "mgmc_fem_stencil": [C.c_int, _vp, C.c_double, _vp],
     "mgmc_contract": [_vp, _vp, _vp, _vp, _vp],
+    "mgmc_contract_noalloc": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],
     "mgmc_autotune": [_vp, C.c_int, C.POINTER(C.c_double)],
     "mgmc_get_tuning": [_vp, C.c_char_p, C.POINTER(C.c_int)],
@@ -101,6 +102,13 @@ def lib():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python -m multigridmc_amd.build` (no CPU fallback exists)")
+    # PyTorch (device tensors handed to the _dev entry points) ships its own
+    # HIP runtime; loading it first makes this library bind to the same one,
+    # so that torch still finds the GPU after the library has initialised it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, args in _SIGS.items():
         f = getattr(L, name)

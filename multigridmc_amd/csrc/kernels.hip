@@ -1655,4 +1655,78 @@ __global__ void __launch_bounds__(256) k_contract(const double* __restrict__ lam
   }
 }
 
+// The contraction of k_contract on vectors in the reference vertex order
+// (interior vertices only, x fastest; src/lattice/lattice3d.hh:132-141), as a
+// caller's device tensors hold them: each (block, chain) walks a grid-stride
+// range of vertices, applies S with Dirichlet boundaries (neighbours outside
+// the interior are zero) and writes its two partial sums to
+// part[(c * gridDim.x + block) * 2 + {0, 1}]. Every partial is overwritten.
+__global__ void __launch_bounds__(256) k_contract_ref(const double* __restrict__ lam, const double* __restrict__ mu,
+                                                      int dim, int nx1, int ny1, int nz1, long long N,
+                                                      StencilT<double> S, double* __restrict__ part) {
+  const int c = blockIdx.y;
+  const double* l = lam + (long long)c * N;
+  const double* u = mu + (long long)c * N;
+  const long long plane = (long long)nx1 * ny1;
+  const int zr = dim == 3 ? 1 : 0;
+  double s0 = 0.0, s1 = 0.0;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(v % nx1);
+    const long long r = v / nx1;
+    const int j = (int)(r % ny1);
+    const int k = (int)(r / ny1);
+    double acc = 0.0;
+    for (int dz = -zr; dz <= zr; ++dz) {
+      if (k + dz < 0 || k + dz >= nz1) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        if (j + dy < 0 || j + dy >= ny1) continue;
+        const double* row = u + v + dz * plane + dy * nx1;
+        const int cb = (dim == 3 ? (dz + 1) * 9 : 0) + (dy + 1) * 3;
+        if (i > 0) acc += S.c[cb] * row[-1];
+        acc += S.c[cb + 1] * row[0];
+        if (i + 1 < nx1) acc += S.c[cb + 2] * row[1];
+      }
+    }
+    s0 += l[v] * u[v];
+    s1 += l[v] * acc;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+  }
+  __shared__ double red[2][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = s0;
+    red[1][wid] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* o = part + ((long long)c * gridDim.x + blockIdx.x) * 2;
+    o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+// out[2c + {0, 1}] = sum over the nblk partials of chain c (one wavefront per
+// chain, lanes take partials lane, lane + 64, ... in order, then a fixed
+// shuffle tree: the same result on every call).
+__global__ void __launch_bounds__(64) k_contract_finish(const double* __restrict__ part, int nblk,
+                                                        double* __restrict__ out) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = lane; b < nblk; b += 64) {
+    s0 += part[((long long)c * nblk + b) * 2];
+    s1 += part[((long long)c * nblk + b) * 2 + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+  }
+  if (lane == 0) {
+    out[2 * c] = s0;
+    out[2 * c + 1] = s1;
+  }
+}
+
 }  // namespace mgmc

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,39 @@ static thread_local std::string g_error;
 static unsigned nblocks(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 void launch_failed(hipError_t e) { throw Error(MGMC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e)); }
+
+// Device memory of mgmc_contract_noalloc: the per-block partial sums of the
+// contraction live in this fixed array instead of a buffer allocated per
+// call. Each GPU the library runs on holds its own copy (512 KiB) until the
+// process ends; host code must use the address hipGetSymbolAddress returns
+// for the current device (scratch_address), never the array's name, which on
+// the host is only a stand-in for the symbol.
+constexpr size_t kScratchDoubles = 65536;
+#ifndef MGMC_EMULATION
+static __device__ __align__(256) double scratch[kScratchDoubles];
+#else
+static double scratch[kScratchDoubles];
+#endif
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {};
+
+// Address of `scratch` on device `dev`, looked up once per device with that
+// device current (mgmc_create calls it at set-up).
+static double* scratch_address(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw Error(MGMC_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+#ifndef MGMC_EMULATION
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double*>(p);
+#else
+    g_scratch[dev] = scratch;
+#endif
+  }
+  return g_scratch[dev];
+}
 
 template <typename X>
 struct DevBuf {
@@ -114,6 +148,8 @@ struct Common {
   mgmc_params p{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  int device = 0;
+  double* scratch = nullptr;  // this device's `scratch` (scratch_address)
   uint32_t k0 = 0, k1 = 0;
   HostHierarchy H;
   std::vector<std::unique_ptr<Level>> L;
@@ -1034,14 +1070,12 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     C.own_stream = true;
   }
 #ifndef MGMC_EMULATION
-  {
-    int dev = 0;
-    HIP_CHECK(hipGetDevice(&dev));
-    HIP_CHECK(hipDeviceGetAttribute(&C.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  }
+  HIP_CHECK(hipGetDevice(&C.device));
+  HIP_CHECK(hipDeviceGetAttribute(&C.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, C.device));
 #else
   C.max_lds = 163840;
 #endif
+  C.scratch = scratch_address(C.device);
   for (int l = 0; l < C.nlevel; ++l) {
     const HostLevel& HL = C.H.levels[l];
     auto Lv = std::make_unique<Level>();
@@ -1475,6 +1509,34 @@ int mgmc_contract(mgmc_handle* h, const double* stencil, const double* lam, cons
     out[2 * c] = a;
     out[2 * c + 1] = b2;
   }
+  API_END
+}
+
+// mgmc_contract on device vectors with no allocation: the partial sums go
+// to this device's `scratch` array (every slot used is overwritten, so
+// nothing needs clearing) and the result to out_dev; both launches are
+// queued on the handle's stream and nothing synchronises, so the call can be
+// captured in a graph. Calls that overlap on one GPU share the array and race.
+int mgmc_contract_noalloc(mgmc_handle* h, const double* stencil, const double* lam_dev, const double* mu_dev,
+                          double* out_dev) {
+  API_BEGIN
+  if (!h || !stencil || !lam_dev || !mu_dev || !out_dev) throw Error(MGMC_ERR_ARG, "null argument");
+  Common& C = h->C;
+  const DevGeom& g = C.L[0]->g;
+  const long long N = C.L[0]->N;
+  const int nblk = (int)std::max(1LL, std::min(256LL, (N + 2047) / 2048));
+  if ((size_t)2 * C.K * nblk > kScratchDoubles)
+    throw Error(MGMC_ERR_ARG, "mgmc_contract_noalloc: " + std::to_string(C.K) + " chains x " + std::to_string(nblk) +
+                                  " blocks need " + std::to_string(2 * (size_t)C.K * nblk) +
+                                  " partial sums; the scratch array holds " + std::to_string(kScratchDoubles));
+  StencilT<double> S{};
+  for (int e = 0; e < 27; ++e) S.c[e] = 0.0;
+  const int ns = g.dim == 3 ? 27 : 9;
+  for (int e = 0; e < ns; ++e) S.c[e] = stencil[e];
+  launch(dim3(nblk, C.K), 256, 0, C.stream, k_contract_ref, lam_dev, mu_dev, g.dim, g.n0 - 1, g.n1 - 1,
+         g.dim == 3 ? g.n2 - 1 : 1, N, S, C.scratch);
+  launch(C.K, 64, 0, C.stream, k_contract_finish, (const double*)C.scratch, nblk, out_dev);
+  HIP_CHECK(hipGetLastError());
   API_END
 }
 

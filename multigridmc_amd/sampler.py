@@ -138,6 +138,12 @@ class _Handle:
         check(_lib.lib().mgmc_contract(self.h, ptr(f64(stencil)), ptr(ll), ptr(mm), ptr(out)))
         return out
 
+    def contract_noalloc(self, stencil, lam_dev, mu_dev, out_dev):
+        """mgmc_contract_noalloc on device pointers (nchains x N reference-order
+        FP64 inputs, 2 x nchains output): allocates nothing and does not
+        synchronise the handle's stream."""
+        check(_lib.lib().mgmc_contract_noalloc(self.h, ptr(f64(stencil)), lam_dev, mu_dev, out_dev))
+
     def ndof(self, level):
         n = C.c_int64()
         check(_lib.lib().mgmc_ndof(self.h, level, C.byref(n)))
