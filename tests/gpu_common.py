@@ -80,3 +80,32 @@ def normals_reference(lat, nchains, tag, sample, seed):
                       rb * np.sin(2 * np.pi * ub)], axis=1)
         out[c] = z[np.arange(N), lane]
     return out
+
+
+# ---- statistical comparison of sampled probe moments with exact ones ----
+def moment_zscores(obs, mean_exact, cov_exact):
+    """obs[nsamples, nchains, nprobe] -> (z of the probe means, z of the
+    covariance entries on and above the diagonal): each deviation from the
+    exact value in units of its Monte Carlo standard error. A mean's error is
+    sqrt(tau_p C_pp / n); a covariance entry, estimated about the exact mean,
+    has error sqrt(tau (C_pp C_qq + C_pq^2) / n) for Gaussian samples, with
+    tau the larger of its probes' integrated autocorrelation times (an upper
+    bound for the product series) and n = nsamples * nchains."""
+    n, K, P = obs.shape
+    taus = np.array([max(1.0, mg.tau_int_chains(obs[:, :, p], mean=mean_exact[p])[0]) for p in range(P)])
+    z = obs.reshape(-1, P)
+    ns = z.shape[0]
+    dg = np.diag(cov_exact)
+    zm = (z.mean(axis=0) - mean_exact) / np.sqrt(dg * taus / ns)
+    d = z - mean_exact
+    C = d.T @ d / ns
+    se = np.sqrt((np.outer(dg, dg) + cov_exact ** 2) * np.maximum.outer(taus, taus) / ns)
+    iu = np.triu_indices(P)
+    return zm, ((C - cov_exact) / se)[iu]
+
+
+def familywise_z(nquantities, alpha=0.01):
+    """Two-sided bound that nquantities standard normal deviates all stay
+    within with probability at least 1 - alpha (Bonferroni)."""
+    from scipy.stats import norm
+    return float(norm.isf(alpha / (2.0 * nquantities)))

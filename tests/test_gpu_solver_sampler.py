@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import multigridmc_amd as mg
-from gpu_common import make_problem, oracle_mg, params
+from gpu_common import familywise_z, make_problem, moment_zscores, oracle_mg, params
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -74,10 +74,13 @@ def _posterior_moments(ora, f, probes):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("smoother,cycle", [("SSOR", 1), ("SOR", 2)])
 def test_mgmc_posterior_moments(precision, smoother, cycle):
-    """Batched MGMC chains sample N(Q^-1 f, Q^-1): means and covariances at
-    probe vertices agree with the exact posterior within 3 Monte Carlo
-    standard errors (the statistical test of test_sampler.hh:113-153; the
-    largest deviations with these seeds are 2.6 and 2.8 errors)."""
+    """Batched MGMC chains sample N(Q^-1 f, Q^-1): the 12 probe means and 78
+    probe covariance entries agree with the exact posterior (dense oracle
+    inverse) within the family-wise bound of their Monte Carlo standard
+    errors (the statistical test of test_sampler.hh:113-153, with the
+    per-quantity 3 standard errors corrected for the 90 quantities tested;
+    each probe's own tau_int sets its errors), and the mean square of the
+    standardised deviations is that of Monte Carlo noise."""
     dim, n = 2, (16, 16)
     op, ora = make_problem(dim, n, 2, radius=0.15, var=1e-3)
     N = op.lattice.Nvertex
@@ -94,16 +97,12 @@ def test_mgmc_posterior_moments(precision, smoother, cycle):
     s.run(nwarm, record=False)
     obs = s.run(ns)
     mu, cov = _posterior_moments(ora, f, probes)
-    tau, _ = mg.tau_int_chains(obs[:, :, 0])
-    neff = K * ns / max(tau, 1.0)
-    m_emp = obs.mean(axis=(0, 1))
-    se_mean = np.sqrt(np.diag(cov) / neff)
-    assert np.all(np.abs(m_emp - mu[probes]) < 3 * se_mean + 1e-6)
-    d = obs.reshape(-1, len(probes)) - m_emp
-    cov_emp = d.T @ d / d.shape[0]
-    se_cov = np.sqrt((np.outer(np.diag(cov), np.diag(cov)) + cov**2) / neff)
-    assert np.all(np.abs(cov_emp - cov) < 3 * se_cov + 1e-6)
-    assert tau < 3.0
+    zm, zc = moment_zscores(obs, mu[probes], cov)
+    worst = max(np.abs(zm).max(), np.abs(zc).max())
+    print(f"largest |z| {worst:.2f}, mean z^2 {np.mean(np.r_[zm, zc] ** 2):.2f}")
+    assert worst < familywise_z(len(zm) + len(zc))
+    assert np.mean(np.r_[zm, zc] ** 2) < 2.0
+    assert mg.tau_int_chains(obs[:, :, 0])[0] < 3.0
 
 
 def test_graph_replay_matches_eager():
