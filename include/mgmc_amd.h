@@ -40,6 +40,7 @@ enum mgmc_status {
 enum mgmc_smoother { MGMC_SOR = 0, MGMC_SSOR = 1 };
 enum mgmc_coarse { MGMC_COARSE_CHOLESKY = 0, MGMC_COARSE_SSOR = 1 };
 enum mgmc_precision { MGMC_FP64 = 0, MGMC_FP32 = 1 };
+enum mgmc_ordering { MGMC_ORDER_MULTICOLOUR = 0, MGMC_ORDER_LEXICOGRAPHIC = 1 };
 
 /* Posterior precision Q = A + B Sigma^{-1} B^T on the fine lattice
  * (src/linear_operator/linear_operator.hh:28-213).
@@ -127,6 +128,18 @@ int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph);
 int mgmc_moments(mgmc_handle* h, int nsamples, double* mean, double* variance);
 /* Reset the sample counter that, with the seed, keys the Philox noise. */
 int mgmc_set_counter(mgmc_handle* h, uint64_t sample_counter);
+
+/* Visiting order of every SOR sweep of the handle (default multicolour).
+ * MGMC_ORDER_LEXICOGRAPHIC is the reference's own order: vertices in
+ * increasing index forward and decreasing index backward
+ * (src/smoother/sor_smoother.cc:56-77, src/sampler/sor_sampler.cc:37-59),
+ * run as wavefronts of independent vertices, with the SMW correction of
+ * that order (src/smoother/sor_smoother.cc:17-51); cycles and solves then
+ * follow the reference's arithmetic and per-cycle residual reduction, at
+ * many more launches per sweep. MGMC_ORDER_MULTICOLOUR (red-black / 2^d
+ * colours) is the fast default; both are exact Gibbs samplers of the same
+ * posterior. */
+int mgmc_set_ordering(mgmc_handle* h, int ordering);
 
 /* MultigridPreconditioner::apply (src/preconditioner/multigrid_preconditioner.cc:104-109):
  * one cycle from a zero initial guess, b, x: nchains x N. */

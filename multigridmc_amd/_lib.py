@@ -86,6 +86,7 @@ _SIGS = {
     "mgmc_contract": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_contract_noalloc": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],
+    "mgmc_set_ordering": [_vp, C.c_int],
     "mgmc_autotune": [_vp, C.c_int, C.POINTER(C.c_double)],
     "mgmc_get_tuning": [_vp, C.c_char_p, C.POINTER(C.c_int)],
     "mgmc_profile": [_vp, C.c_int],

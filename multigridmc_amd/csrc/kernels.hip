@@ -362,6 +362,60 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
 }
 
 // ---------------------------------------------------------------------------
+// One wavefront of a lexicographic SOR sweep: the reference's own visiting
+// order (SORSmoother::apply_sparse, src/smoother/sor_smoother.cc:56-77,
+// vertices in increasing index order forward, decreasing backward) run in
+// parallel. With hyperplanes s = i + ay j + az k, where (ay, az) = (1, 1) for
+// star stencils and (2, 4) for box stencils, every neighbour that precedes a
+// vertex in index order lies on a lower hyperplane and every later one on a
+// higher hyperplane, so updating hyperplane s after s - 1 (forward; s + 1
+// backward) sees exactly the old and new values the sequential sweep sees.
+// The arithmetic follows the reference's: res = sum of the row's entries times
+// x in increasing column order (zero stencil entries and boundary slots add
+// exact zeros), x += omega (c - res) / D, c = f + sqrt(D (2 - omega) / omega) xi.
+// A thread updates one vertex; (j, k, chain) come from the thread index, i
+// from s. The Philox normal of a vertex is the one the multicolour kernels
+// draw for it (lane (i & 7) >> 1 of its group's call).
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, int NOISE>
+__global__ void __launch_bounds__(256) k_sweep_lex(T* __restrict__ x, const T* __restrict__ f,
+                                                   const T* __restrict__ ext, DevGeom g, Stencil st, int s, int ay,
+                                                   int az, int K, NoiseKey nk) {
+  const int nj = g.n1 - 1, nk3 = DIM == 3 ? g.n2 - 1 : 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nj * nk3 * K) return;
+  const int j = (int)(t % nj) + 1;
+  const long long r = t / nj;
+  const int k = DIM == 3 ? (int)(r % nk3) + 1 : 0;
+  const int chain = (int)(r / nk3);
+  const int i = s - ay * j - az * k;
+  if (i < 1 || i > g.n0 - 1) return;
+  const long long off = ((long long)k * g.py + j) * g.px + i;
+  T* xp = x + chain * g.cs + off;
+  const long long pl = (long long)g.px * g.py;
+  T res = (T)0;
+#pragma unroll
+  for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const double a = st.c[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        if (a != 0.0) res += (T)a * xp[dz * pl + dy * g.px + dx];
+      }
+  T c = f[chain * g.fcs + off];
+  if (NOISE == NOISE_PHILOX) {
+    float xi[4];
+    const int ngx = g.px >> 3;
+    const uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + (i >> 3)) << 1) | (uint32_t)(i & 1);
+    normals4(c0, (uint32_t)chain, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+    c += (T)st.noise_scale * (T)xi[(i & 7) >> 1];
+  }
+  if (NOISE == NOISE_EXTERNAL) c += (T)st.noise_scale * ext[chain * g.cs + off];
+  *xp += (T)st.omega * (c - res) / (T)st.diag;
+}
+
+// ---------------------------------------------------------------------------
 // Low-rank (measurement) terms, in two kernels.
 //
 // k_lr_dot: block (k, c) forms z[c][k] = sum_e B_ek x_c[idx_e] (B^T x, FP64
@@ -752,6 +806,12 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_copy(T* __restrict__ dst, const T* __restrict__ src, long long n) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = src[t];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_add(T* __restrict__ dst, const T* __restrict__ src, long long n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] += src[t];
 }
 
 template <typename T>

@@ -76,9 +76,20 @@ struct DevBuf {
     n = count;
     if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(X)));
   }
-  void upload(const X* h, size_t count) {
+  // Host data into a fresh buffer. Uploads after set-up pass the handle's
+  // stream: the copy is ordered on it and waited for (the handles' streams
+  // do not synchronise with the null stream). Without a stream (set-up) the
+  // device is synchronised after the copy.
+  void upload(const X* h, size_t count, hipStream_t s = nullptr) {
     alloc(count);
-    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice));
+    if (!count) return;
+    if (s) {
+      HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(X), hipMemcpyHostToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      HIP_CHECK(hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice));
+      HIP_CHECK(hipDeviceSynchronize());
+    }
   }
   void free() {
     if (p) (void)hipFree(p);
@@ -134,6 +145,7 @@ struct Level {
   DevRows Brows;
   DevRows W[2];
   DevBuf<double> Sinv[2];
+  DevBuf<double> SinvLex[2];  // capacitance inverses for the lexicographic order (mgmc_set_ordering)
 };
 
 // k_lr_dot: B columns are split into up to kMaxLrSplit blocks of about
@@ -181,6 +193,7 @@ struct Common {
   int tune_planes = 1;        // plane-parity sweeps on 3D box levels
   int tune_planes_min = 1;    // ... when planes x chains reaches this many workgroups
   int tune_planes_stage3 = 1; // ... with planes k-1, k, k+1 staged in LDS when they fit
+  int ordering = MGMC_ORDER_MULTICOLOUR;  // visiting order of the SOR sweeps (mgmc_set_ordering)
   // optional HIP-event timing of the level-0 colour-phase launches
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -209,6 +222,7 @@ struct EngineBase {
   virtual void set_f_from_fp64(const double* pad64) = 0;
   virtual void add_x_to_fp64(double* pad64) = 0;
   virtual void retune() = 0;
+  virtual void set_ordering(int ordering) = 0;
   virtual bool f_shared() const = 0;
   virtual void accumulate_moments(double* m1, double* m2) = 0;
   virtual void unpack_moment(const double* m, double* ref_dev, double scale) = 0;
@@ -242,6 +256,7 @@ struct Engine : EngineBase {
   std::vector<int> cur;
   std::vector<DevBuf<T>> coef;  // the level's 27 stencil coefficients in precision T
   Field<T> ext;  // external noise staging (test hooks)
+  std::vector<Field<T>> lw;     // lexicographic order: the SMW correction T^{-1} B u
 
   // x of level l is logically zero but not yet written (zero_x_lazy): the
   // first plane-parity sweep or the coarse sample absorbs it; every other
@@ -270,6 +285,7 @@ struct Engine : EngineBase {
     f.resize(C.nlevel);
     r.resize(C.nlevel);
     x2.resize(C.nlevel);
+    lw.resize(C.nlevel);
     cur.assign(C.nlevel, 0);
     xzero.assign(C.nlevel, 0);
     pending.assign(C.nlevel, 0);
@@ -307,7 +323,7 @@ struct Engine : EngineBase {
   }
   bool fused_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (!C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || !C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
     return rb_ty(l) >= std::min(4, Lv.g.n1 - 1);
   }
   RbShape rb_shape(int l) const {
@@ -395,7 +411,7 @@ struct Engine : EngineBase {
   }
   bool march_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (!C.tune_march || Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || !C.tune_march || Lv.star || Lv.g.dim != 3) return false;
     const MarchShape m = march_shape(l);
     return m.threads <= 1024 && m.lds <= (size_t)C.max_lds;
   }
@@ -444,7 +460,7 @@ struct Engine : EngineBase {
 
   bool planes_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (!C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || !C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
     const DevGeom& g = Lv.g;
     // one workgroup per (plane, chain): worth it only with enough workgroups
     return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= (size_t)C.max_lds &&
@@ -621,12 +637,76 @@ struct Engine : EngineBase {
 #undef MGMC_LRSU
   }
 
+  // ---- lexicographic visiting order (k_sweep_lex) ----
+  bool lex() const { return C.ordering == MGMC_ORDER_LEXICOGRAPHIC; }
+  // one directional sweep of x with right-hand side f as wavefronts
+  // s = i + ay j + az k, in increasing s forward and decreasing s backward
+  void lex_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag, T* xp, const T* fp, DevGeom gg) {
+    const Level& Lv = *C.L[l];
+    const DevGeom& g = Lv.g;
+    const int ay = Lv.star ? 1 : 2, az = g.dim == 3 ? (Lv.star ? 1 : 4) : 0;
+    const int smin = 1 + ay + az;
+    const int smax = (g.n0 - 1) + ay * (g.n1 - 1) + az * (g.n2 - 1);
+    const long long threads = (long long)(g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1) * C.K;
+    const NoiseKey nk = key(tag);
+    const bool timed = prof_begin(l);
+    for (int q = 0; q <= smax - smin; ++q) {
+      const int s = forward ? smin + q : smax - q;
+#define MGMC_LEX(DIM, NOISE) \
+  launch(nblocks(threads), 256, 0, C.stream, k_sweep_lex<T, DIM, NOISE>, xp, fp, extp, gg, Lv.st, s, ay, az, C.K, nk)
+      if (g.dim == 3) {
+        if (noise == NOISE_NONE) MGMC_LEX(3, NOISE_NONE);
+        else if (noise == NOISE_PHILOX) MGMC_LEX(3, NOISE_PHILOX);
+        else MGMC_LEX(3, NOISE_EXTERNAL);
+      } else {
+        if (noise == NOISE_NONE) MGMC_LEX(2, NOISE_NONE);
+        else if (noise == NOISE_PHILOX) MGMC_LEX(2, NOISE_PHILOX);
+        else MGMC_LEX(2, NOISE_EXTERNAL);
+      }
+#undef MGMC_LEX
+    }
+    if (timed) prof_end();
+  }
+  // the SMW correction in lexicographic order: x += T^{-1} (B u) with u from
+  // the last lr_dot_solve, T^{-1} v being one sweep of a zero field with
+  // right-hand side v (the reference's dense B_bar = T^{-1} B S^{-1} applied
+  // without storing it: T^{-1} B fills every vertex after a ball)
+  void lex_correct(int l, bool forward) {
+    Level& Lv = *C.L[l];
+    const long long n = (long long)Lv.g.cs * C.K;
+    fill(r[l].p, n, (T)0);
+    lr_apply(r[l].p, Lv.g.cs, Lv.Brows);
+    fill(lw[l].p, n, (T)0);
+    DevGeom gg = Lv.g;  // per-chain right-hand side
+    lex_sweep(l, forward, NOISE_NONE, nullptr, 0, lw[l].p, r[l].p, gg);
+    launch(nblocks(n), 256, 0, C.stream, k_add<T>, X(l), (const T*)lw[l].p, n);
+  }
+  void set_ordering(int ordering) override {
+    C.ordering = ordering;
+    if (!lex()) return;
+    for (int l = 0; l < C.nlevel; ++l) {
+      if (C.m > 0 && !lw[l].p) lw[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
+      // normalise(): the state of levels swept out of place is back in x
+    }
+    HIP_CHECK(hipStreamSynchronize(C.stream));
+  }
+
   // One directional SOR sweep (+ SMW), SORSmoother::apply / SORSampler::apply
   // with nsmooth = 1. `sparse_repeats` > 1 reproduces the nsmooth^2 sparse
   // sweeps of SORSmoother with nsmooth > 1 (src/smoother/sor_smoother.cc:44,63).
   void dir_sweep(int l, bool forward, int noise, const T* extp, const double* extlr, int sparse_repeats = 1) {
     Level& Lv = *C.L[l];
     uint32_t tag = C.tag++;
+    if (lex()) {
+      for (int s = 0; s < sparse_repeats; ++s)
+        lex_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag, X(l), f[l].p, geo(l));
+      if (C.m > 0) {
+        const int d = forward ? 1 : 0;
+        lr_dot_solve(l, X(l), Lv.SinvLex[d].p, noise, extlr, tag);
+        lex_correct(l, forward);
+      }
+      return;
+    }
     for (int s = 0; s < sparse_repeats; ++s) {
       if (fused_ok(l)) {
         fused_sweep(l, forward, s == 0 ? noise : NOISE_NONE, extp, tag);
@@ -1266,9 +1346,9 @@ static void set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, con
     idx[e] = HL.g.padded(vertices[e]);
     w[e] = weights ? weights[e] : 1.0;
   }
-  h->C.obs_ptr.upload(ptr.data(), ptr.size());
-  h->C.obs_idx.upload(idx.data(), idx.size());
-  h->C.obs_w.upload(w.data(), w.size());
+  h->C.obs_ptr.upload(ptr.data(), ptr.size(), h->C.stream);
+  h->C.obs_idx.upload(idx.data(), idx.size(), h->C.stream);
+  h->C.obs_w.upload(w.data(), w.size(), h->C.stream);
   h->C.nprobe = nobs;
   h->graph_valid = 0;
 }
@@ -1553,7 +1633,7 @@ int mgmc_sweep(mgmc_handle* h, int level, int forward, int stochastic, const dou
   h->E->pack(level, 0, dx, 1);
   double* dxi = xi ? stage(h->io3, xi, n, C.stream) : nullptr;
   DevBuf<double> lr;
-  if (xi_lr && C.m > 0) lr.upload(xi_lr, (size_t)C.m * C.K);
+  if (xi_lr && C.m > 0) lr.upload(xi_lr, (size_t)C.m * C.K, C.stream);
   C.tag = 1;
   h->E->sweep(level, forward != 0, stochastic != 0, dxi, xi_lr && C.m > 0 ? lr.p : nullptr);
   h->E->unpack(level, 0, dx, 1.0);
@@ -1623,6 +1703,24 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out) {
   launch(nblocks(n), 256, 0, C.stream, k_normals, d, C.L[0]->g, C.K, nk);
   HIP_CHECK(hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
   sync(h);
+  API_END
+}
+
+int mgmc_set_ordering(mgmc_handle* h, int ordering) {
+  API_BEGIN
+  if (!h) throw Error(MGMC_ERR_ARG, "null handle");
+  if (ordering != MGMC_ORDER_MULTICOLOUR && ordering != MGMC_ORDER_LEXICOGRAPHIC)
+    throw Error(MGMC_ERR_ARG, "invalid ordering");
+  Common& C = h->C;
+  if (ordering == MGMC_ORDER_LEXICOGRAPHIC && C.m > 0 && !C.L[0]->SinvLex[0].p) {
+    for (int l = 0; l < C.nlevel; ++l) {
+      std::vector<double> S[2];
+      lex_capacitance(C.H.levels[l], C.H.omega, C.H.sigma, S);
+      for (int d = 0; d < 2; ++d) C.L[l]->SinvLex[d].upload(S[d].data(), S[d].size(), C.stream);
+    }
+  }
+  h->E->set_ordering(ordering);
+  h->graph_valid = 0;
   API_END
 }
 

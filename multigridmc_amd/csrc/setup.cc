@@ -392,6 +392,54 @@ static void compute_lowrank(HostLevel& L, const std::vector<Column>& cols, const
   }
 }
 
+// (Sigma + B^T T^{-1} B)^{-1} for the lexicographic visiting order, where T
+// = D/omega + the couplings to the vertices visited before (earlier index
+// forward, dir = 1; later index backward, dir = 0): the capacitance matrix of
+// SORSmoother's SMW correction (src/smoother/sor_smoother.cc:17-37) in the
+// reference's own order. Each column w = T^{-1} b_k is one forward (backward)
+// substitution over the whole level, as the reference's triangular solve.
+void lex_capacitance(const HostLevel& L, double omega, const std::vector<double>& sigma, std::vector<double> out[2]) {
+  const Geom& g = L.g;
+  const int m = L.m;
+  const int zr = g.dim == 3 ? 1 : 0;
+  const int64_t pl = (int64_t)g.px * g.py;
+  // the box plus one plane, row and slot: when px = n_x the (+1, +1, +1)
+  // neighbour of the last vertex is the slot after the box (zero, as the
+  // padding after a chain's box is on the device)
+  const size_t len = (size_t)g.vol + (size_t)pl + (size_t)g.px + 1;
+  std::vector<double> b(len), w(len);
+  const double dw = L.diag / omega;
+  for (int dir = 0; dir < 2; ++dir) {
+    // offsets of the neighbours visited before a vertex, increasing index order
+    std::vector<std::pair<int64_t, double>> before;
+    for (int dz = -zr; dz <= zr; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const double a = L.st[sidx(dx, dy, dz)];
+          const bool earlier = dz < 0 || (dz == 0 && (dy < 0 || (dy == 0 && dx < 0)));
+          if (a != 0.0 && (dx || dy || dz) && earlier == (dir == 1)) before.push_back({dz * pl + dy * g.px + dx, a});
+        }
+    std::vector<double> S((size_t)m * m, 0.0);
+    for (int l = 0; l < m; ++l) {
+      std::fill(b.begin(), b.end(), 0.0);
+      std::fill(w.begin(), w.end(), 0.0);
+      for (int32_t e = L.bptr[l]; e < L.bptr[l + 1]; ++e) b[L.bidx[e]] += L.bval[e];
+      const int64_t nvtx = g.N;
+      for (int64_t q = 0; q < nvtx; ++q) {
+        const int64_t v = dir == 1 ? q : nvtx - 1 - q;
+        const int64_t p = g.padded(v);
+        double acc = b[p];
+        for (auto& o : before) acc -= o.second * w[p + o.first];
+        w[p] = acc / dw;
+      }
+      for (int k = 0; k < m; ++k)
+        for (int32_t e = L.bptr[k]; e < L.bptr[k + 1]; ++e) S[(size_t)k * m + l] += L.bval[e] * w[L.bidx[e]];
+    }
+    for (int k = 0; k < m; ++k) S[(size_t)k * m + k] += sigma[k];
+    out[dir] = inverse(S, m);
+  }
+}
+
 static void build_coarse(const HostLevel& L, const std::vector<Column>& cols, const std::vector<double>& sigma,
                          HostCoarse& C) {
   const Geom& g = L.g;

@@ -111,6 +111,11 @@ constexpr int64_t kMaxCoarseKernel = 2048;
 // (the factor is then required); otherwise it is built when it fits.
 HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bool coarse_cholesky = true);
 
+// (Sigma + B^T T^{-1} B)^{-1} of a level for the lexicographic visiting order
+// ([0] backward, [1] forward; m x m row-major), T = D/omega + the couplings to
+// the vertices visited before.
+void lex_capacitance(const HostLevel& L, double omega, const std::vector<double>& sigma, std::vector<double> out[2]);
+
 // Galerkin coarse stencil of a constant fine stencil with linear interpolation.
 void galerkin_stencil(int dim, const double* fine, double* coarse);
 

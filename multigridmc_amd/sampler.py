@@ -67,12 +67,17 @@ _COARSE = {"Cholesky": 0, "SSOR": 1}
 _PREC = {"fp64": 0, "fp32": 1}
 
 
+_ORDERING = {"multicolour": 0, "lexicographic": 1}
+
+
 class _Handle:
-    def __init__(self, op, params, nchains, precision, seed, stream=None):
+    def __init__(self, op, params, nchains, precision, seed, stream=None, ordering="multicolour"):
         if params.smoother not in _SMOOTHERS:
             raise _lib.MGMCError(1, f"invalid smoother '{params.smoother}'")
         if params.coarse_solver not in _COARSE:
             raise _lib.MGMCError(1, f"invalid coarse solver '{params.coarse_solver}'")
+        if ordering not in _ORDERING:
+            raise _lib.MGMCError(1, f"invalid ordering '{ordering}'")
         L = _lib.lib()
         prob, self._keep = _problem(op)
         par = _lib.Params()
@@ -92,6 +97,9 @@ class _Handle:
         h = C.c_void_p()
         check(L.mgmc_create(C.byref(prob), C.byref(par), C.byref(h)))
         self.h = h
+        if ordering != "multicolour":
+            check(L.mgmc_set_ordering(h, _ORDERING[ordering]))
+        self.ordering = ordering
         self.nchains = nchains
         self.precision = precision
         self.op = op
@@ -161,10 +169,13 @@ class MultigridMCSampler(_Handle):
     """MGMC sampler (src/sampler/multigridmc_sampler.cc:8-145) for K batched chains.
 
     apply(f, x) draws one new sample per chain: x has shape (nchains, N) (or (N,)
-    for a single chain) and is updated in place, as Sampler::apply does."""
+    for a single chain) and is updated in place, as Sampler::apply does.
+    ordering="lexicographic" sweeps in the reference's own vertex order
+    (wavefronts, mgmc_set_ordering) instead of the fast multicolour order."""
 
-    def __init__(self, linear_operator, params=None, nchains=1, precision="fp64", seed=0x5eed, stream=None):
-        super().__init__(linear_operator, params or MultigridParameters(), nchains, precision, seed, stream)
+    def __init__(self, linear_operator, params=None, nchains=1, precision="fp64", seed=0x5eed, stream=None,
+                 ordering="multicolour"):
+        super().__init__(linear_operator, params or MultigridParameters(), nchains, precision, seed, stream, ordering)
 
     def get_linear_operator(self):
         return self.op
@@ -296,8 +307,8 @@ class SSORSampler(MultigridMCSampler):
 class MultigridPreconditioner(_Handle):
     """Zero-initial-guess multigrid cycle (src/preconditioner/multigrid_preconditioner.cc:8-109)."""
 
-    def __init__(self, linear_operator, params=None, precision="fp64", nchains=1, stream=None):
-        super().__init__(linear_operator, params or MultigridParameters(), nchains, precision, 0, stream)
+    def __init__(self, linear_operator, params=None, precision="fp64", nchains=1, stream=None, ordering="multicolour"):
+        super().__init__(linear_operator, params or MultigridParameters(), nchains, precision, 0, stream, ordering)
 
     def apply(self, b, x=None):
         bb = self._rows(b)
