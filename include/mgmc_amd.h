@@ -3,8 +3,10 @@
  * The library implements the MultigridMC hot path behind plain C entry
  * points: the multigrid Monte Carlo (MGMC) sampler and its zero-noise twin,
  * the multigrid preconditioner with the Richardson loop solver. Host code
- * above this ABI (the C++ adapters in multigridmc_amd/csrc/adapters and the
- * Python package multigridmc_amd) mirrors the reference's class interfaces.
+ * above this ABI mirrors the reference's class interfaces: the C++ classes
+ * of multigridmc_amd/csrc/adapters/mgmc_adapters.hh (Sampler,
+ * HipMultigridMCSampler, Preconditioner, HipMultigridPreconditioner,
+ * HipLoopSolver; libmgmc_adapters.so) and the Python package multigridmc_amd.
  *
  * Conventions
  *  - Every vector handed across the ABI is FP64 in the reference's vertex

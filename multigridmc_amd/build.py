@@ -32,8 +32,34 @@ def build(verbose=True):
     return LIB
 
 
+ADAPTERS = os.path.join(HERE, "libmgmc_adapters.so")
+
+
+def build_adapters(verbose=True):
+    """libmgmc_adapters.so: the C++ classes with the reference's Sampler /
+    Preconditioner / LoopSolver shapes over the C ABI, and bin/adapter_demo."""
+    src = os.path.join(CSRC, "adapters")
+    c = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", os.path.join(src, "mgmc_adapters.cc"),
+         "-L" + HERE, "-lmgmc_amd", "-Wl,-rpath,$ORIGIN", "-o", ADAPTERS + ".tmp"]
+    if verbose:
+        print(" ".join(c), flush=True)
+    subprocess.check_call(c)
+    os.replace(ADAPTERS + ".tmp", ADAPTERS)
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "adapter_demo")
+    c = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(src, "adapter_demo.cc"), "-L" + HERE,
+         "-lmgmc_adapters", "-lmgmc_amd", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(c), flush=True)
+    subprocess.check_call(c)
+    os.replace(out + ".tmp", out)
+
+
 def build_drivers(verbose=True):
-    """driver_mgmc and driver_mg (host C++ over the C ABI) into multigridmc_amd/bin."""
+    """driver_mgmc and driver_mg (host C++ over the C ABI; driver_mg through
+    the adapter classes) into multigridmc_amd/bin."""
+    build_adapters(verbose)
     host = os.path.join(CSRC, "host")
     bindir = os.path.join(HERE, "bin")
     os.makedirs(bindir, exist_ok=True)
@@ -41,7 +67,7 @@ def build_drivers(verbose=True):
     for name in ("driver_mgmc", "driver_mg"):
         out = os.path.join(bindir, name)
         c = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(host, name + ".cc")] + common + [
-            "-L" + HERE, "-lmgmc_amd", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+            "-L" + HERE, "-lmgmc_adapters", "-lmgmc_amd", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
         if verbose:
             print(" ".join(c), flush=True)
         subprocess.check_call(c)

@@ -214,11 +214,14 @@ void GpuParameters::parse(const Setting& root) {
     if (g.exists("convergence_chains")) convergence_chains = (int)g.lookup("convergence_chains").as_int();
     if (g.exists("precision")) precision = g.lookup("precision").as_string();
     if (g.exists("seed")) seed = (uint64_t)g.lookup("seed").as_int();
+    if (g.exists("ordering")) ordering = g.lookup("ordering").as_string();
   });
   if (precision != "fp64" && precision != "fp32") die("gpu.precision has to be 'fp64' or 'fp32'");
+  if (ordering != "multicolour" && ordering != "lexicographic")
+    die("gpu.ordering has to be 'multicolour' or 'lexicographic'");
   if (nchains < 1 || convergence_chains < 1) die("gpu.nchains and gpu.convergence_chains must be >= 1");
   std::cout << "  gpu: chains = " << nchains << ", convergence chains per batch = " << convergence_chains
-            << ", precision = " << precision << ", seed = " << seed << std::endl;
+            << ", precision = " << precision << ", ordering = " << ordering << ", seed = " << seed << std::endl;
 }
 
 // ---------------------------------------------------------------------------

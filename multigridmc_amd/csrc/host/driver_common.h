@@ -88,6 +88,7 @@ struct GpuParameters {
   int nchains = 1;               // chains of the timed sampling run
   int convergence_chains = 256;  // chains per batch of the convergence / MSE runs
   std::string precision = "fp64";
+  std::string ordering = "multicolour";  // or "lexicographic": the reference's sweep order
   uint64_t seed = 1;
   void parse(const Setting& root);
 };

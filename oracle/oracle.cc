@@ -33,6 +33,7 @@
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <random>
 #endif
 
 namespace orc {
@@ -1390,6 +1391,15 @@ void orc_rng_uniform(orc_rng* r, long n, double* out) { for (long i = 0; i < n; 
 void orc_rng_normal(orc_rng* r, long n, double* out) { for (long i = 0; i < n; ++i) out[i] = r->g.draw_normal(); }
 
 double orc_tau_int_reference(const double* q, long n, int window) { return tau_int_reference(q, n, window); }
+
+// The right-hand side of driver_mg (src/driver_mg.cc:163-171): n draws of
+// std::normal_distribution<double>(0, 1) from std::mt19937_64(1482817), with
+// the standard library the drivers are built against.
+void orc_driver_mg_rhs(long n, double* out) {
+  std::mt19937_64 rng(1482817);
+  std::normal_distribution<double> normal_dist(0.0, 1.0);
+  for (long i = 0; i < n; ++i) out[i] = normal_dist(rng);
+}
 
 int orc_num_threads() {
 #ifdef _OPENMP

@@ -78,6 +78,7 @@ def lib():
         "orc_rng_normal": (None, [vp, C.c_long, _dp]),
         "orc_tau_int_reference": (C.c_double, [_dp, C.c_long, C.c_int]),
         "orc_num_threads": (C.c_int, []),
+        "orc_driver_mg_rhs": (None, [C.c_long, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -355,6 +356,13 @@ class CLCG:
         out = np.zeros(n)
         lib().orc_rng_normal(self.h, n, out)
         return out
+
+
+def driver_mg_rhs(n):
+    """driver_mg's right-hand side: std::normal_distribution draws from std::mt19937_64(1482817)."""
+    out = np.zeros(n)
+    lib().orc_driver_mg_rhs(n, out)
+    return out
 
 
 def tau_int_reference(series, window):

@@ -70,9 +70,7 @@ def test_driver_mgmc_outputs_and_statistics(tmp_path):
     assert (tmp_path / "sample_location.vtk").exists()
 
 
-def test_driver_mg_solves_and_reports(tmp_path):
-    rc, out = run_driver("driver_mg", tmp_path)
-    assert rc == 0, out[-3000:]
+def _residual_table(out):
     lines = out.splitlines()
     i = lines.index("%5s   %8s   %12s   %6s" % ("iter", "||r||", "||r||/||r_0||", "rho"))
     rows = []
@@ -80,10 +78,56 @@ def test_driver_mg_solves_and_reports(tmp_path):
         parts = ln.split()
         if len(parts) != 4 or not parts[0].isdigit():
             break
-        rows.append([float(v) for v in parts])
-    rows = np.array(rows)
+        rows.append(parts)
+    return rows
+
+
+def _oracle_solve(ordering):
+    """The oracle's loop solver on driver_mg's problem and right-hand side
+    (the same libstdc++ std::mt19937_64(1482817) normal draws)."""
+    from oracle import pyoracle as po
+    op, _ = _problem()
+    ora = po.Operator.fd(2, (32, 32), 1.0 / 0.3 ** 2)
+    ora.add_measurements(np.array([[0.25, 0.25], [0.25, 0.75], [0.75, 0.25], [0.75, 0.75]]), 0.05,
+                         np.array([1.0e-3, 2.0e-3, 1.5e-3, 1.0e-3]))
+    omg = po.Multigrid(ora, 3, ordering=ordering)
+    b = po.driver_mg_rhs(op.lattice.Nvertex)
+    return omg.loop_solve(b, rtol=1e-10, atol=1e-12, maxiter=50)
+
+
+@pytest.mark.parametrize("ordering", ["multicolour", "lexicographic"])
+def test_driver_mg_matches_oracle_solve(tmp_path, ordering):
+    """driver_mg (through HipMultigridPreconditioner and HipLoopSolver) prints
+    the residual table of the oracle's loop solver on the same right-hand
+    side, digit for digit down to residuals of 1e-9 relative (multicolour: the oracle's multicolour twin;
+    lexicographic, gpu.ordering: the reference's own order, so the table is
+    the reference's), and writes its solution to solution.vtk."""
+    from oracle import pyoracle as po
+    cfg = fixture_text("posterior_2d.cfg").replace("gpu = {", f'gpu = {{\n    ordering = "{ordering}";', 1)
+    rc, out = run_driver("driver_mg", tmp_path, cfg_text=cfg)
+    assert rc == 0, out[-3000:]
+    rows = _residual_table(out)
+    x_ref, hist, it = _oracle_solve(po.LEX if ordering == "lexicographic" else po.MULTICOLOUR)
+    assert it >= 0 and len(rows) == it + 1
+    # digit for digit while the residual is well above FP64 rounding of its
+    # evaluation (relative 1e-9); below that within rounding of the residual
+    big = hist / hist[0] > 1e-9
+    assert [r[1] for r, g in zip(rows, big) if g] == ["%8.3e" % h for h, g in zip(hist, big) if g]
+    assert np.allclose([float(r[1]) for r in rows], hist, rtol=2e-3, atol=1e-12 * hist[0])
+    assert any(ln.startswith("Solver converged after %5d iterations" % it) for ln in out.splitlines())
+    x = _read_vtk(tmp_path / "solution.vtk")["numerical"].reshape(33, 33)[1:-1, 1:-1].reshape(-1)
+    assert np.abs(x - x_ref).max() <= 1e-5 * np.abs(x_ref).max()
+    if ordering == "lexicographic":
+        rho = np.array([float(r[3]) for r in rows[1:]])
+        print("driver_mg lexicographic rho:", rho)
+
+
+def test_driver_mg_solves_and_reports(tmp_path):
+    rc, out = run_driver("driver_mg", tmp_path)
+    assert rc == 0, out[-3000:]
+    rows = np.array([[float(v) for v in r] for r in _residual_table(out)])
     assert rows[-1, 2] < 1e-10 and np.all(rows[2:, 3] < 0.3)
-    assert any(ln.startswith("Solver converged after") for ln in lines)
+    assert any(ln.startswith("Solver converged after") for ln in out.splitlines())
     x = _read_vtk(tmp_path / "solution.vtk")["numerical"].reshape(33, 33)[1:-1, 1:-1].reshape(-1)
     # the solution solves Q x = b for the driver's standard-normal right-hand side
     op, _ = _problem()
