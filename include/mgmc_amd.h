@@ -155,6 +155,18 @@ int mg_precondition(mgmc_handle* h, const double* b, double* x);
 int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
              int* niter);
 
+/* mg_solve with b and x in device memory (nchains x N, reference order),
+ * e.g. torch tensors; returns when x is written. */
+int mg_solve_dev(mgmc_handle* h, const double* b_dev, double* x_dev, double rtol, double atol, int maxiter,
+                 double* hist, int* niter);
+
+/* The measurement operator on device vectors in reference order
+ * (MeasuredOperator's B, src/linear_operator/measured_operator.cc:9-49):
+ * z_dev (nchains x m) = B^T x_dev (nchains x N), and y_dev (nchains x N) =
+ * B w_dev (nchains x m). Both return when the output is written. */
+int mgmc_measurement_dot_dev(mgmc_handle* h, const double* x_dev, double* z_dev);
+int mgmc_measurement_apply_dev(mgmc_handle* h, const double* w_dev, double* y_dev);
+
 /* LinearOperator::apply (src/linear_operator/linear_operator.hh:78-88) on
  * level 0 in FP64, all chains: y = Q x. */
 int mgmc_apply_operator(mgmc_handle* h, const double* x, double* y);

@@ -75,6 +75,9 @@ _SIGS = {
     "mg_precondition": [_vp, _vp, _vp],
     "mg_solve": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],
     "mgmc_apply_operator": [_vp, _vp, _vp],
+    "mg_solve_dev": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],
+    "mgmc_measurement_dot_dev": [_vp, _vp, _vp],
+    "mgmc_measurement_apply_dev": [_vp, _vp, _vp],
     "mgmc_sweep": [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
     "mgmc_residual_restrict": [_vp, C.c_int, _vp, _vp, _vp],
     "mgmc_prolongate_add": [_vp, C.c_int, C.c_double, _vp, _vp],

@@ -15,15 +15,19 @@ second batched solve lam_r = Q^{-1} g_r with g_r = dL/dmu_r, and
     dL/dy_r     = Sigma^{-1} B^T lam_r
     dL/dsigma2_j = -sum_r (b_j^T lam_r)(y_rj - b_j^T mu_r) / sigma2_j^2
 
-The two field contractions run in the HIP kernel k_contract (mgmc_contract);
-the m-vectors B^T lam, B^T mu are sparse products on the host. Only the fine
-fields mu are kept between forward and backward.
+Everything stays in device memory: B Sigma^{-1} y, the solves, B^T lam and
+B^T mu run on the library's device entry points (mgmc_measurement_apply_dev,
+mg_solve_dev, mgmc_measurement_dot_dev) and the two field contractions in the
+HIP kernel of mgmc_contract_noalloc; torch holds the tensors. Only the fine
+fields mu and the handle built in the forward pass are kept for the backward
+pass (the hierarchy is not rebuilt). Tensors on the CPU are moved to the GPU
+and the results returned on the caller's device.
 """
 import numpy as np
 import torch
 
 from .operators import MeasuredOperator, ShiftedLaplaceFDOperator
-from .sampler import LoopSolver, MultigridParameters, MultigridPreconditioner
+from .sampler import MultigridParameters, MultigridPreconditioner
 
 
 class PosteriorMeanProblem:
@@ -47,13 +51,13 @@ class PosteriorMeanProblem:
         prior = ShiftedLaplaceFDOperator(self.lattice, float(kappa_sq), diffusion=float(s))
         return MeasuredOperator(prior, self.centres, self.radius, np.asarray(sigma2, dtype=np.float64), B=self.B)
 
-    def solve(self, op, rhs):
-        """Q^{-1} rhs for each row; returns (solution, the handle used)."""
-        rhs = np.atleast_2d(rhs)
-        pre = MultigridPreconditioner(op, self.params, precision=self.precision, nchains=rhs.shape[0])
-        solver = LoopSolver(op, pre, rtol=self.rtol, atol=1e300, maxiter=self.maxiter)
-        x = np.asarray(solver.apply(rhs)).reshape(rhs.shape)
-        return x, pre
+    def solve_dev(self, handle, rhs, x):
+        """x = Q^{-1} rhs row by row for device tensors (R x N, FP64) with the
+        handle's multigrid loop solver (stopping on the relative residual)."""
+        it = handle.solve_dev(rhs.data_ptr(), x.data_ptr(), self.rtol, 1e300, self.maxiter)
+        if it < 0:
+            raise RuntimeError(f"posterior-mean solve did not reach rtol {self.rtol} in {self.maxiter} iterations")
+        return x
 
 
 def _nlevel(lattice):
@@ -65,37 +69,56 @@ def _nlevel(lattice):
     return L
 
 
+def _ptr(t):
+    return t.data_ptr()
+
+
 class _PosteriorMean(torch.autograd.Function):
     @staticmethod
     def forward(ctx, kappa_sq, s, y, sigma2, problem):
+        dev = torch.device("cuda")
         k, sv = float(kappa_sq), float(s)
-        yy = y.detach().cpu().numpy().astype(np.float64)
-        s2 = sigma2.detach().cpu().numpy().astype(np.float64)
-        op = problem.operator(k, sv, s2)
-        rhs = (problem.B @ (yy / s2).T).T
-        mu, _ = problem.solve(op, rhs)
-        ctx.problem = problem
-        ctx.op = op
-        ctx.mu = mu
-        ctx.yy = yy
-        ctx.s2 = s2
-        return torch.from_numpy(mu.copy()).to(y.dtype)
+        # Sigma enters the hierarchy set-up on the host (m values)
+        s2_host = sigma2.detach().to("cpu", torch.float64).numpy()
+        op = problem.operator(k, sv, s2_host)
+        yd = y.detach().to(dev, torch.float64).contiguous()
+        s2d = sigma2.detach().to(dev, torch.float64).contiguous()
+        R = yd.shape[0]
+        N = problem.lattice.Nvertex
+        h = MultigridPreconditioner(op, problem.params, precision=problem.precision, nchains=R)
+        w = (yd / s2d).contiguous()
+        rhs = torch.empty((R, N), dtype=torch.float64, device=dev)
+        mu = torch.empty((R, N), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        h.measurement_apply_dev(_ptr(w), _ptr(rhs))
+        problem.solve_dev(h, rhs, mu)
+        ctx.problem, ctx.handle, ctx.op = problem, h, op
+        ctx.save_for_backward(mu, yd, s2d)
+        ctx.in_device = y.device
+        return mu.to(y.device, y.dtype)
 
     @staticmethod
     def backward(ctx, grad_mu):
-        problem, op, mu, yy, s2 = ctx.problem, ctx.op, ctx.mu, ctx.yy, ctx.s2
-        g = grad_mu.detach().cpu().numpy().astype(np.float64).reshape(mu.shape)
-        lam, handle = problem.solve(op, g)
-        dots = handle.contract(op.base.laplacian_stencil(), lam, mu)  # (lam.mu, lam^T K mu) per row
-        d_kappa = -problem.cell_volume * dots[:, 0].sum()
-        d_s = -dots[:, 1].sum()
-        Btl = (problem.B.T @ lam.T).T  # (R, m)
-        Btm = (problem.B.T @ mu.T).T
-        d_y = Btl / s2
-        d_sigma2 = -(Btl * (yy - Btm)).sum(axis=0) / (s2 * s2)
-        t = grad_mu.dtype
-        return (torch.tensor(d_kappa, dtype=t), torch.tensor(d_s, dtype=t), torch.from_numpy(d_y).to(t),
-                torch.from_numpy(d_sigma2).to(t), None)
+        problem, h, op = ctx.problem, ctx.handle, ctx.op
+        mu, yd, s2d = ctx.saved_tensors
+        dev = mu.device
+        R, m = yd.shape
+        g = grad_mu.detach().to(dev, torch.float64).contiguous()
+        lam = torch.empty_like(mu)
+        dots = torch.empty(2 * R, dtype=torch.float64, device=dev)
+        Btl = torch.empty((R, m), dtype=torch.float64, device=dev)
+        Btm = torch.empty((R, m), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        problem.solve_dev(h, g, lam)
+        h.contract_noalloc(op.base.laplacian_stencil(), _ptr(lam), _ptr(mu), _ptr(dots))  # (lam.mu, lam^T K mu)
+        h.measurement_dot_dev(_ptr(lam), _ptr(Btl))  # synchronises the handle's stream
+        h.measurement_dot_dev(_ptr(mu), _ptr(Btm))
+        d_kappa = -problem.cell_volume * dots[0::2].sum()
+        d_s = -dots[1::2].sum()
+        d_y = Btl / s2d
+        d_sigma2 = -(Btl * (yd - Btm)).sum(dim=0) / (s2d * s2d)
+        t, out = grad_mu.dtype, ctx.in_device
+        return (d_kappa.to(out, t), d_s.to(out, t), d_y.to(out, t), d_sigma2.to(out, t), None)
 
 
 def posterior_mean(kappa_sq, s, y, sigma2, problem):

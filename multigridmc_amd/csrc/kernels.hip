@@ -1715,6 +1715,51 @@ __global__ void __launch_bounds__(256) k_contract(const double* __restrict__ lam
   }
 }
 
+// Reference vertex index of a padded-layout index (interior vertices only;
+// src/lattice/lattice3d.hh:132-141).
+__device__ __forceinline__ long long padded_to_ref(long long p, const DevGeom& g) {
+  const int i = (int)(p % g.px);
+  const long long q = p / g.px;
+  const int j = (int)(q % g.py);
+  const int k = (int)(q / g.py);
+  const long long row = g.dim == 3 ? (long long)(k - 1) * (g.n1 - 1) + (j - 1) : (long long)(j - 1);
+  return row * (g.n0 - 1) + (i - 1);
+}
+
+// z[c][k] = (B^T x_c)_k for x in reference order (nchains x N): block (k, c)
+// walks column k's entries (padded indices, converted) with a fixed
+// reduction order.
+__global__ void __launch_bounds__(256) k_bt_ref(const double* __restrict__ x, long long N,
+                                                const int32_t* __restrict__ bptr, const long long* __restrict__ bidx,
+                                                const double* __restrict__ bval, DevGeom g, int m,
+                                                double* __restrict__ z) {
+  const int kc = blockIdx.x, c = blockIdx.y;
+  const double* xc = x + (long long)c * N;
+  double s = 0.0;
+  for (int e = bptr[kc] + threadIdx.x; e < bptr[kc + 1]; e += blockDim.x) s += bval[e] * xc[padded_to_ref(bidx[e], g)];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) z[(long long)c * m + kc] = red[0] + red[1] + red[2] + red[3];
+}
+
+// y_c[row] = sum_e val_e w_c[col_e] over the rows of B (padded row indices,
+// converted), for w (nchains x m); y (nchains x N in reference order) is
+// zero elsewhere (cleared before the launch).
+__global__ void __launch_bounds__(256) k_b_ref(const double* __restrict__ w, const long long* __restrict__ rows,
+                                               const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                               const double* __restrict__ val, int nrows, DevGeom g, long long N,
+                                               int m, double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= nrows) return;
+  double s = 0.0;
+  for (int e = ptr[r]; e < ptr[r + 1]; ++e) s += val[e] * w[(long long)c * m + col[e]];
+  y[(long long)c * N + padded_to_ref(rows[r], g)] = s;
+}
+
 // The contraction of k_contract on vectors in the reference vertex order
 // (interior vertices only, x fastest; src/lattice/lattice3d.hh:132-141), as a
 // caller's device tensors hold them: each (block, chain) walks a grid-stride

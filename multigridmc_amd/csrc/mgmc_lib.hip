@@ -1492,9 +1492,9 @@ int mgmc_apply_operator(mgmc_handle* h, const double* x, double* y) {
 
 // LoopSolver::apply (src/solver/loop_solver.cc:9-54) on all chains at once:
 // the stopping test must hold for every chain; hist records chain 0.
-int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
-             int* niter) {
-  API_BEGIN
+// b_dev / x_dev: reference order, nchains x N, device memory.
+static void loop_solve(mgmc_handle* h, const double* b_dev, double* x_dev, double rtol, double atol, int maxiter,
+                       double* hist, int* niter) {
   Common& C = h->C;
   int64_t N = C.L[0]->N;
   size_t n = (size_t)N * C.K;
@@ -1507,8 +1507,7 @@ int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double ato
   HIP_CHECK(hipMemsetAsync(h->x64.p, 0, np * sizeof(double), C.stream));
   HIP_CHECK(hipMemsetAsync(h->r64.p, 0, np * sizeof(double), C.stream));
   HIP_CHECK(hipMemsetAsync(h->b64.p, 0, np * sizeof(double), C.stream));
-  double* db = stage(h->io1, b, n, C.stream);
-  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, db, h->b64.p, C.L[0]->g, C.K, 1);
+  launch(nblocks(n), 256, 0, C.stream, k_pack<double>, b_dev, h->b64.p, C.L[0]->g, C.K, 1);
   const int nb = 128;
   if (h->sums.n < (size_t)nb * C.K) h->sums.alloc((size_t)nb * C.K);
   std::vector<double> part((size_t)nb * C.K);
@@ -1542,10 +1541,65 @@ int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double ato
     h->E->add_x_to_fp64(h->x64.p);
   }
   if (niter) *niter = it;
-  launch(nblocks(n), 256, 0, C.stream, k_unpack<double>, h->x64.p, db, C.L[0]->g, C.K, 1.0);
-  HIP_CHECK(hipMemcpyAsync(x, db, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  launch(nblocks(n), 256, 0, C.stream, k_unpack<double>, h->x64.p, x_dev, C.L[0]->g, C.K, 1.0);
   sync(h);
   if (it < 0) throw Error(MGMC_ERR_NOCONV, "solver failed to converge after maxiter iterations");
+}
+
+int mg_solve(mgmc_handle* h, const double* b, double* x, double rtol, double atol, int maxiter, double* hist,
+             int* niter) {
+  API_BEGIN
+  Common& C = h->C;
+  const size_t n = (size_t)C.L[0]->N * C.K;
+  double* db = stage(h->io1, b, n, C.stream);
+  double* dx = stage(h->io2, nullptr, n, C.stream);
+  int code = MGMC_OK;
+  std::string msg;
+  try {
+    loop_solve(h, db, dx, rtol, atol, maxiter, hist, niter);
+  } catch (const Error& e) {
+    if (e.code != MGMC_ERR_NOCONV) throw;
+    code = e.code;
+    msg = e.what();
+  }
+  HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  if (code != MGMC_OK) throw Error(code, msg);
+  API_END
+}
+
+int mg_solve_dev(mgmc_handle* h, const double* b_dev, double* x_dev, double rtol, double atol, int maxiter,
+                 double* hist, int* niter) {
+  API_BEGIN
+  if (!h || !b_dev || !x_dev) throw Error(MGMC_ERR_ARG, "null argument");
+  loop_solve(h, b_dev, x_dev, rtol, atol, maxiter, hist, niter);
+  API_END
+}
+
+int mgmc_measurement_dot_dev(mgmc_handle* h, const double* x_dev, double* z_dev) {
+  API_BEGIN
+  if (!h || !x_dev || !z_dev) throw Error(MGMC_ERR_ARG, "null argument");
+  Common& C = h->C;
+  if (C.m == 0) throw Error(MGMC_ERR_STATE, "the operator has no measurements");
+  const Level& L0 = *C.L[0];
+  launch(dim3(C.m, C.K), 256, 0, C.stream, k_bt_ref, x_dev, (long long)L0.N, L0.bptr.p, L0.bidx.p, L0.bval.p, L0.g,
+         C.m, z_dev);
+  sync(h);
+  API_END
+}
+
+int mgmc_measurement_apply_dev(mgmc_handle* h, const double* w_dev, double* y_dev) {
+  API_BEGIN
+  if (!h || !w_dev || !y_dev) throw Error(MGMC_ERR_ARG, "null argument");
+  Common& C = h->C;
+  if (C.m == 0) throw Error(MGMC_ERR_STATE, "the operator has no measurements");
+  const Level& L0 = *C.L[0];
+  const long long n = (long long)L0.N * C.K;
+  launch(nblocks(n), 256, 0, C.stream, k_fill<double>, y_dev, n, 0.0);
+  if (L0.Brows.nrows)
+    launch(dim3(nblocks(L0.Brows.nrows), C.K), 256, 0, C.stream, k_b_ref, w_dev, L0.Brows.row.p, L0.Brows.ptr.p,
+           L0.Brows.col.p, L0.Brows.val.p, L0.Brows.nrows, L0.g, (long long)L0.N, C.m, y_dev);
+  sync(h);
   API_END
 }
 

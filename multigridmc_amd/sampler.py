@@ -146,6 +146,23 @@ class _Handle:
         check(_lib.lib().mgmc_contract(self.h, ptr(f64(stencil)), ptr(ll), ptr(mm), ptr(out)))
         return out
 
+    def solve_dev(self, b_dev, x_dev, rtol, atol, maxiter):
+        """mg_solve_dev on device pointers (nchains x N each); returns the
+        converged iteration (-1 if maxiter was reached without converging)."""
+        it = C.c_int(-1)
+        code = _lib.lib().mg_solve_dev(self.h, b_dev, x_dev, rtol, atol, maxiter, None, C.byref(it))
+        if code != 5:
+            check(code)
+        return it.value
+
+    def measurement_dot_dev(self, x_dev, z_dev):
+        """z = B^T x for device pointers (nchains x N -> nchains x m)."""
+        check(_lib.lib().mgmc_measurement_dot_dev(self.h, x_dev, z_dev))
+
+    def measurement_apply_dev(self, w_dev, y_dev):
+        """y = B w for device pointers (nchains x m -> nchains x N)."""
+        check(_lib.lib().mgmc_measurement_apply_dev(self.h, w_dev, y_dev))
+
     def contract_noalloc(self, stencil, lam_dev, mu_dev, out_dev):
         """mgmc_contract_noalloc on device pointers (nchains x N reference-order
         FP64 inputs, 2 x nchains output): allocates nothing and does not
