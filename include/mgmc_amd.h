@@ -203,6 +203,15 @@ int mgmc_normals(mgmc_handle* h, uint32_t tag, double* out);
  * (MGMC_ERR_ARG if it exceeds `capacity`). n holds dim cell counts. */
 int mgmc_measurement_vector(int dim, const int* n, const double* x0, double radius, int64_t capacity, int64_t* idx,
                             double* val, int64_t* nnz);
+/* The measurement vector with a choice of weight: profile 0 is the top-hat
+ * of mgmc_measurement_vector; profile 1 weights the ball average with
+ * exp(-xi^2 / (2 width^2)), xi = |x - x0| / radius < 1, normalised to
+ * integral one over the ball (a Gaussian-weighted ball average; no
+ * counterpart in the reference, whose weight f_meas is 1,
+ * src/linear_operator/measured_operator.hh:65). Same quadrature and output
+ * as mgmc_measurement_vector. */
+int mgmc_measurement_vector_weighted(int dim, const int* n, const double* x0, double radius, int profile,
+                                     double width, int64_t capacity, int64_t* idx, double* val, int64_t* nnz);
 /* Constant 3^d stencil of the Q1 finite-element prior -Delta + kappa^2
  * (src/linear_operator/shiftedlaplace_fem_operator.cc:9-141), index
  * (dz+1)*9 + (dy+1)*3 + (dx+1) (3D) or (dy+1)*3 + (dx+1) (2D); building A as a

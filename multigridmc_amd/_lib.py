@@ -86,6 +86,8 @@ _SIGS = {
     "mgmc_level_stencil": [_vp, C.c_int, _vp, C.POINTER(C.c_int)],
     "mgmc_measurement_vector": [C.c_int, _vp, _vp, C.c_double, C.c_int64, _vp, _vp, C.POINTER(C.c_int64)],
     "mgmc_fem_stencil": [C.c_int, _vp, C.c_double, _vp],
+    "mgmc_measurement_vector_weighted": [C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_double, C.c_int64, _vp, _vp,
+                                         C.POINTER(C.c_int64)],
     "mgmc_contract": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_contract_noalloc": [_vp, _vp, _vp, _vp, _vp],
     "mgmc_set_tuning": [_vp, C.c_char_p, C.c_int],

@@ -38,7 +38,7 @@ std::shared_ptr<LinearOperator> LinearOperator::from_csr(const Lattice& lattice,
 std::shared_ptr<LinearOperator> LinearOperator::measured(const std::shared_ptr<LinearOperator>& prior,
                                                          const std::vector<double>& centres, double radius,
                                                          const std::vector<double>& variance,
-                                                         double variance_scaling) {
+                                                         double variance_scaling, int profile, double width) {
   const Lattice& lat = prior->lattice;
   const int m = (int)variance.size();
   if ((int)centres.size() != m * lat.dim) throw Error(MGMC_ERR_ARG, "measured: centres must hold m x dim values");
@@ -54,9 +54,8 @@ std::shared_ptr<LinearOperator> LinearOperator::measured(const std::shared_ptr<L
     for (;;) {
       idx.resize((size_t)cap);
       val.resize((size_t)cap);
-      const int st =
-          mgmc_measurement_vector(lat.dim, lat.n, centres.data() + (size_t)k * lat.dim, radius, cap, idx.data(),
-                                  val.data(), &nnz);
+      const int st = mgmc_measurement_vector_weighted(lat.dim, lat.n, centres.data() + (size_t)k * lat.dim, radius,
+                                                      profile, width, cap, idx.data(), val.data(), &nnz);
       if (st == MGMC_OK) break;
       if (st == MGMC_ERR_ARG && nnz > cap) {
         cap = nnz;

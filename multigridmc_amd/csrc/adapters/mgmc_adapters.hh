@@ -115,9 +115,12 @@ class LinearOperator {
   // MeasuredOperator (src/linear_operator/measured_operator.cc:9-49): the
   // prior plus ball-average measurements at `centres` (m x dim), Sigma =
   // variance_scaling * variance.
+  // profile 0: the reference's top-hat weight; 1: Gaussian weight of the
+  // given width (mgmc_measurement_vector_weighted).
   static std::shared_ptr<LinearOperator> measured(const std::shared_ptr<LinearOperator>& prior,
                                                   const std::vector<double>& centres, double radius,
-                                                  const std::vector<double>& variance, double variance_scaling = 1.0);
+                                                  const std::vector<double>& variance, double variance_scaling = 1.0,
+                                                  int profile = 0, double width = 0.5);
   // The prior plus m measurements given as B (N x m CSR, reference vertex
   // order) and Sigma (variances already scaled).
   static std::shared_ptr<LinearOperator> with_measurements(const std::shared_ptr<LinearOperator>& prior, int m,

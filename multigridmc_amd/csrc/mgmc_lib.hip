@@ -1256,6 +1256,22 @@ int mgmc_measurement_vector(int dim, const int* n, const double* x0, double radi
   API_END
 }
 
+int mgmc_measurement_vector_weighted(int dim, const int* n, const double* x0, double radius, int profile,
+                                     double width, int64_t capacity, int64_t* idx, double* val, int64_t* nnz) {
+  API_BEGIN
+  if (!n || !x0 || !nnz) throw Error(MGMC_ERR_ARG, "null argument");
+  for (int d = 0; d < dim && d < 3; ++d)
+    if (n[d] < 2) throw Error(MGMC_ERR_ARG, "lattice needs at least 2 cells per direction");
+  std::vector<int64_t> vi;
+  std::vector<double> vv;
+  measurement_vector(dim, n, x0, radius, vi, vv, profile, width);
+  *nnz = (int64_t)vi.size();
+  if ((int64_t)vi.size() > capacity) throw Error(MGMC_ERR_ARG, "capacity too small for the measurement vector");
+  std::copy(vi.begin(), vi.end(), idx);
+  std::copy(vv.begin(), vv.end(), val);
+  API_END
+}
+
 int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil) {
   API_BEGIN
   if (!n || !stencil) throw Error(MGMC_ERR_ARG, "null argument");

@@ -27,14 +27,39 @@ static int64_t vertex_linear(int dim, const int* n, const int* e) {
   return ell;
 }
 
-// w_j = int f(x) phi_j(x) dx for the top-hat f of integral one on the ball
+// Integral of exp(-|u|^2 / (2 w^2)) over the unit d-ball (radial
+// integration). 2D: 2 pi w^2 (1 - e^{-1/(2w^2)}), with expm1. 3D: 4 pi
+// int_0^1 u^2 e^{-u^2/(2w^2)} du, which is w^3 (sqrt(pi/2) erf(1/(sqrt(2) w))
+// - e^{-1/(2w^2)} / w) (by parts) for w < 1 and the series
+// sum_k (-1)^k / ((2w^2)^k k! (2k + 3)) for w >= 1, where the closed form
+// cancels.
+double gaussian_ball_integral(double w, int dim) {
+  const double a = 0.5 / (w * w);
+  if (dim == 2) return -2.0 * M_PI * w * w * std::expm1(-a);
+  if (w < 1.0)
+    return 4.0 * M_PI * w * w * w * (std::sqrt(0.5 * M_PI) * std::erf(1.0 / (std::sqrt(2.0) * w)) - std::exp(-a) / w);
+  double sum = 0.0, term = 1.0;  // term = (-a)^k / k!
+  for (int k = 0; k < 40; ++k) {
+    sum += term / (2 * k + 3);
+    term *= -a / (k + 1);
+  }
+  return 4.0 * M_PI * sum;
+}
+
+// w_j = int f(x) phi_j(x) dx for a weight f of integral one on the ball
 // (x0, radius), 2^d-point Gauss-Legendre per cell; the nearest vertex for
 // radius < 1e-12 (src/linear_operator/measured_operator.cc:69-170). Cells are
 // visited in increasing linear order inside a bounding box of the ball, a
 // cell counts when one of its corners lies inside the ball or it contains
 // x0, and contributions accumulate cell by cell, corner by corner.
+// profile 0: the reference's top-hat f = 1 / V_d(R) (f_meas = 1,
+// measured_operator.hh:65); profile 1: f = exp(-xi^2 / (2 width^2)) / (R^d c_d)
+// with xi = |x - x0| / R < 1 and c_d = gaussian_ball_integral(width, d), a
+// Gaussian-weighted ball average (no counterpart in the reference).
 void measurement_vector(int dim, const int* n, const double* x0, double radius, std::vector<int64_t>& idx,
-                        std::vector<double>& val) {
+                        std::vector<double>& val, int profile, double width) {
+  if (profile != 0 && profile != 1) throw Error(1, "measurement vector: profile must be 0 (top-hat) or 1 (Gaussian)");
+  if (profile == 1 && !(width > 0.0)) throw Error(1, "measurement vector: the Gaussian width must be positive");
   idx.clear();
   val.clear();
   if (dim < 2 || dim > 3) throw Error(1, "measurement vector: dim must be 2 or 3");
@@ -66,7 +91,8 @@ void measurement_vector(int dim, const int* n, const double* x0, double radius, 
     val.push_back(1.0);
     return;
   }
-  const double norm = 1.0 / ball_volume(radius, dim);
+  const double norm =
+      profile == 0 ? 1.0 / ball_volume(radius, dim) : 1.0 / (std::pow(radius, dim) * gaussian_ball_integral(width, dim));
   const int nq = 1 << dim;
   const double g1[2] = {0.5 * (-1.0 / std::sqrt(3.0) + 1.0), 0.5 * (1.0 / std::sqrt(3.0) + 1.0)};
   const double qw = std::pow(0.5, dim);
@@ -97,13 +123,16 @@ void measurement_vector(int dim, const int* n, const double* x0, double radius, 
           contains = contains && h[d] * cell[d] <= x0[d] && x0[d] <= h[d] * (cell[d] + 1);
         if (!(overlap || contains)) continue;
         bool inside[8];
+        double fq[8];
         for (int q = 0; q < nq; ++q) {
           double s = 0.0;
           for (int d = 0; d < dim; ++d) {
             const double xq = h[d] * (g1[bit(q, d)] + cell[d]);
             s += (xq - x0[d]) * (xq - x0[d]);
           }
-          inside[q] = std::sqrt(s) / radius < 1.0;
+          const double xi = std::sqrt(s) / radius;
+          inside[q] = xi < 1.0;
+          fq[q] = profile == 0 ? 1.0 : std::exp(-0.5 * xi * xi / (width * width));
         }
         for (int a = 0; a < nq; ++a) {
           int v[3] = {1, 1, 1};
@@ -114,7 +143,7 @@ void measurement_vector(int dim, const int* n, const double* x0, double radius, 
           }
           double local = 0.0;
           for (int q = 0; q < nq; ++q) {
-            double phihat = 1.0;
+            double phihat = fq[q];
             for (int d = 0; d < dim; ++d) {
               const double xq = g1[bit(q, d)];
               phihat *= bit(a, d) == 0 ? 1.0 - xq : xq;

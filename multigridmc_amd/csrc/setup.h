@@ -99,8 +99,11 @@ struct HostHierarchy {
 
 // Ball-average measurement vector (reference vertex indices ascending, values)
 // and the Q1 FEM stencil; operators.cc.
+// profile 0: top-hat (the reference's), 1: Gaussian weight of the given width
+// (standard deviation in units of the radius), both of integral one on the ball.
 void measurement_vector(int dim, const int* n, const double* x0, double radius, std::vector<int64_t>& idx,
-                        std::vector<double>& val);
+                        std::vector<double>& val, int profile = 0, double width = 0.5);
+double gaussian_ball_integral(double width, int dim);
 void fem_stencil(int dim, const int* n, double kappa_sq, double* st);
 
 // Largest coarsest level (unknowns) for the dense Cholesky factor; the

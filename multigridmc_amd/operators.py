@@ -224,11 +224,40 @@ def V_sphere(radius, dim):
     return 2.0 * math.pi / dim * radius * radius * V_sphere(radius, dim - 2)
 
 
-def measurement_vector(lattice, x0, radius):
-    """Dual-space ball average w_j = int f(x) phi_j(x) dx with top-hat f of
+def gaussian_ball_integral(width, dim):
+    """Integral of exp(-|u|^2 / (2 width^2)) over the unit d-ball (d = 2, 3):
+    2 pi w^2 (1 - e^{-1/(2w^2)}) in 2D; in 3D 4 pi int_0^1 u^2
+    e^{-u^2/(2w^2)} du, in closed form for w < 1 and as its power series for
+    w >= 1 (where the closed form cancels)."""
+    w = float(width)
+    a = 0.5 / (w * w)
+    if dim == 2:
+        return -2.0 * math.pi * w * w * math.expm1(-a)
+    if w < 1.0:
+        return 4.0 * math.pi * w * w * w * (math.sqrt(0.5 * math.pi) * math.erf(1.0 / (math.sqrt(2.0) * w))
+                                            - math.exp(-a) / w)
+    total, term = 0.0, 1.0
+    for k in range(40):
+        total += term / (2 * k + 3)
+        term *= -a / (k + 1)
+    return 4.0 * math.pi * total
+
+
+PROFILES = {"tophat": 0, "gaussian": 1}
+
+
+def measurement_vector(lattice, x0, radius, profile="tophat", width=0.5):
+    """Dual-space ball average w_j = int f(x) phi_j(x) dx with a weight f of
     integral one over the ball of `radius` around x0, 2^d-point Gauss-Legendre
     per cell (src/linear_operator/measured_operator.cc:69-170); radius < 1e-12
-    selects the nearest vertex (:74-90). Returns (vertex indices, values)."""
+    selects the nearest vertex (:74-90). Returns (vertex indices, values).
+
+    profile "tophat": the reference's f = 1 / V_d(R); "gaussian": f =
+    exp(-xi^2 / (2 width^2)) / (R^d c_d), xi = |x - x0| / R < 1, c_d its
+    integral over the unit ball (a Gaussian-weighted ball average, no
+    counterpart in the reference)."""
+    if profile not in PROFILES:
+        raise ValueError(f"unknown measurement profile '{profile}'")
     L = lattice
     dim = L.dim
     x0 = np.asarray(x0, dtype=np.float64)
@@ -238,7 +267,10 @@ def measurement_vector(lattice, x0, radius):
         return np.array([int(np.argmin(dist))]), np.array([1.0])
     h = np.array([1.0 / v for v in L.n])
     cv = L.cell_volume()
-    norm = 1.0 / V_sphere(radius, dim)
+    if profile == "tophat":
+        norm = 1.0 / V_sphere(radius, dim)
+    else:
+        norm = 1.0 / (radius ** dim * gaussian_ball_integral(width, dim))
     # corners / quadrature points: Cartesian products with the first component slowest
     bits = np.array([[(c >> (dim - 1 - d)) & 1 for d in range(dim)] for c in range(1 << dim)])
     p1 = np.array([0.5 * (-1.0 / math.sqrt(3.0) + 1.0), 0.5 * (1.0 / math.sqrt(3.0) + 1.0)])
@@ -257,13 +289,18 @@ def measurement_vector(lattice, x0, radius):
     overlap |= np.all((cmin <= x0) & (x0 <= cmax), axis=1)
     cells = cells[overlap]
     xq = h * (qp[None, :, :] + cells[:, None, :])  # (ncell, nq, dim)
-    inside = np.sqrt(((xq - x0) ** 2).sum(axis=2)) / radius < 1.0
+    xi = np.sqrt(((xq - x0) ** 2).sum(axis=2)) / radius
+    inside = xi < 1.0
+    fq = np.ones_like(xi) if profile == "tophat" else np.exp(-0.5 * xi * xi / (width * width))
     rows, vals = [], []
     for a in range(1 << dim):
         vert = cells + bits[a]
         ok = np.all((vert >= 1) & (vert <= np.array(L.n) - 1), axis=1)
-        phihat = np.prod(np.where(bits[a] == 0, 1.0 - qp, qp), axis=1)  # (nq,)
-        contrib = np.where(inside, phihat[None, :] * qw * cv * norm, 0.0)
+        # the weight times the corner's basis function, factors in dimension order
+        phihat = fq.copy()
+        for d in range(dim):
+            phihat = phihat * np.where(bits[a, d] == 0, 1.0 - qp[:, d], qp[:, d])[None, :]
+        contrib = np.where(inside, phihat * qw * cv * norm, 0.0)
         # sum over quadrature points in the reference's order
         local = np.zeros(len(cells))
         for q in range(contrib.shape[1]):
@@ -286,9 +323,11 @@ class MeasuredOperator:
     measurement matrix and Sigma = variance_scaling * variance
     (src/linear_operator/measured_operator.cc:9-49)."""
 
-    def __init__(self, base, centres, radius, variance, variance_scaling=1.0, B=None):
+    def __init__(self, base, centres, radius, variance, variance_scaling=1.0, B=None, profile="tophat", width=0.5):
         """B: an already computed measurement matrix for these centres and
-        radius (reused when only the variances or the prior change)."""
+        radius (reused when only the variances or the prior change).
+        profile / width: the ball weight of measurement_vector ("tophat", the
+        reference's, or "gaussian")."""
         self.base = base
         self.lattice = base.lattice
         self.kappa_sq = base.kappa_sq
@@ -303,7 +342,7 @@ class MeasuredOperator:
             return
         rows, cols, vals = [], [], []
         for k in range(m):
-            r, v = measurement_vector(self.lattice, centres[k], radius)
+            r, v = measurement_vector(self.lattice, centres[k], radius, profile, width)
             rows.append(r)
             cols.append(np.full(len(r), k))
             vals.append(v)

@@ -126,3 +126,63 @@ def test_two_point_gauss_rule_is_exact_to_degree_three():
     for k in range(4):
         assert abs(0.5 * np.sum(g ** k) - 1.0 / (k + 1)) < 1e-15
     assert abs(0.5 * np.sum(g ** 4) - 0.2) > 1e-3
+
+
+# ---- Gaussian-weighted balls (a beyond-reference option, pinned by its
+# normalisation, its moments and the reduction to the top-hat) ----
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("width", [0.2, 0.5, 0.99, 1.0, 3.0, 1e3])
+def test_gaussian_ball_integral(dim, width):
+    from scipy.integrate import quad
+    from multigridmc_amd.operators import gaussian_ball_integral
+    surface = 2 * math.pi if dim == 2 else 4 * math.pi
+    ref = surface * quad(lambda u: u ** (dim - 1) * math.exp(-u * u / (2 * width * width)), 0.0, 1.0,
+                         epsabs=0, epsrel=1e-13)[0]
+    assert abs(gaussian_ball_integral(width, dim) - ref) <= 1e-12 * ref
+
+
+def _gauss_moments_weighted(n, x0, R, width):
+    from multigridmc_amd.operators import gaussian_ball_integral
+    dim = len(n)
+    h = 1.0 / np.array(n, dtype=float)
+    g = np.array([0.5 - 0.5 / math.sqrt(3.0), 0.5 + 0.5 / math.sqrt(3.0)])
+    axes = [(np.arange(n[d])[:, None] + g[None, :]).reshape(-1) * h[d] for d in range(dim)]
+    P = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, dim)
+    xi = np.linalg.norm(P - x0, axis=1) / R
+    f = np.where(xi < 1.0, np.exp(-0.5 * xi * xi / width ** 2), 0.0)
+    w = np.prod(h) / 2 ** dim / (R ** dim * gaussian_ball_integral(width, dim))
+    return w * f.sum(), w * (f[:, None] * P).sum(axis=0)
+
+
+@pytest.mark.parametrize("n,x0,R", CASES[:4])
+def test_gaussian_measurement_vector_moments_and_reduction(n, x0, R):
+    from multigridmc_amd import _lib
+    lat = mg.Lattice(*n)
+    x0 = np.array(x0)
+    idx, val = mg.measurement_vector(lat, x0, R, profile="gaussian", width=0.4)
+    X = lat.vertex_coordinates(idx)
+    m0, m1 = _gauss_moments_weighted(n, x0, R, 0.4)
+    assert abs(val.sum() - m0) <= 1e-13 * m0
+    assert np.abs(val @ X - m1).max() <= 1e-13 * np.abs(m1).max()
+    # the C entry point gives the same vector (exp may differ in the last ulp)
+    nn = np.array(list(n) + [1] * (3 - len(n)), dtype=np.int32)
+    cap = 1 << 16
+    ci, cv, nnz = np.zeros(cap, np.int64), np.zeros(cap), C.c_int64()
+    _lib.check(_lib.lib().mgmc_measurement_vector_weighted(lat.dim, nn.ctypes.data, x0.ctypes.data, R, 1, 0.4, cap,
+                                                           ci.ctypes.data, cv.ctypes.data, C.byref(nnz)))
+    assert np.array_equal(ci[:nnz.value], idx)
+    assert np.abs(cv[:nnz.value] - val).max() <= 1e-15 * np.abs(val).max()
+    # a very wide Gaussian is the top-hat
+    it, vt = mg.measurement_vector(lat, x0, R)
+    iw, vw = mg.measurement_vector(lat, x0, R, profile="gaussian", width=1e7)
+    assert np.array_equal(it, iw) and np.abs(vw - vt).max() <= 1e-13 * np.abs(vt).max()
+
+
+def test_gaussian_measurement_vector_normalised():
+    """sum_j b_j -> 1 at second order as h -> 0 (ball away from the boundary)."""
+    errs = []
+    for n in (32, 64, 128):
+        _, val = mg.measurement_vector(mg.Lattice(n, n, n), (0.43, 0.51, 0.47), 0.1, profile="gaussian", width=0.5)
+        errs.append(abs(val.sum() - 1.0))
+    assert errs[2] < 2e-3 and errs[2] < errs[0] / 4
