@@ -46,10 +46,16 @@ enum mgmc_ordering { MGMC_ORDER_MULTICOLOUR = 0, MGMC_ORDER_LEXICOGRAPHIC = 1 };
 
 /* Posterior precision Q = A + B Sigma^{-1} B^T on the fine lattice
  * (src/linear_operator/linear_operator.hh:28-213).
- * A: if A_rowptr is NULL the library assembles ShiftedLaplaceFDOperator with
- * constant kappa^2 (src/linear_operator/shiftedlaplace_fd_operator.cc:9-57);
- * otherwise A is taken as given (CSR, symmetric), and must be a constant-
- * coefficient nearest-neighbour stencil with Dirichlet elimination.
+ * A: if A_rowptr is NULL the library assembles the finite-difference operator
+ * -div(a grad) + kappa^2 scaled by the cell volume
+ * (ShiftedLaplaceFDOperator, src/linear_operator/shiftedlaplace_fd_operator.cc:9-57):
+ * kappa^2 is kappa_sq, or per vertex kappa_sq_field (reference order; the
+ * reference's correlation-length models, src/linear_operator/correlationlength_model.hh);
+ * a is 1, or per cell `diffusion` (prod n_d values, cell (i, j, k) at
+ * (k n_y + j) n_x + i), the coupling across an edge being the mean of a over
+ * the cells sharing it. Otherwise A is taken as given (CSR, symmetric,
+ * nearest-neighbour couplings, Dirichlet elimination); a constant stencil
+ * runs on the fast kernels, varying rows on per-vertex stencils.
  * B: N x m CSR (column k = measurement k), Sigma: m variances (already
  * multiplied by variance_scaling, src/linear_operator/measured_operator.cc:19). */
 typedef struct {
@@ -64,6 +70,8 @@ typedef struct {
   const int64_t* B_col;
   const double* B_val;
   const double* sigma;
+  const double* diffusion;       /* NULL: a = 1 */
+  const double* kappa_sq_field;  /* NULL: kappa_sq everywhere */
 } mgmc_problem;
 
 /* MultigridParameters (src/auxilliary/parameters.hh:150-179) plus the
@@ -217,6 +225,15 @@ int mgmc_measurement_vector_weighted(int dim, const int* n, const double* x0, do
  * (dz+1)*9 + (dy+1)*3 + (dx+1) (3D) or (dy+1)*3 + (dx+1) (2D); building A as a
  * CSR matrix from it and passing it to mgmc_create gives the FEM prior. */
 int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil);
+
+/* Host-side set-up only (no device work): the stencil of every vertex of
+ * `level` of the hierarchy of `problem` (its A, or its coefficient fields;
+ * measurements are ignored) with nlevel levels: out (N_level x 27, reference
+ * vertex order) receives the coupling to each neighbour at index
+ * (dz+1)*9 + (dy+1)*3 + (dx+1), zero where the neighbour is not interior.
+ * For variable coefficients the coarse stencils are the Galerkin products
+ * R A R^T (src/linear_operator/linear_operator.cc:10-23) formed per vertex. */
+int mgmc_level_stencils(const mgmc_problem* problem, int nlevel, int level, double* out);
 
 /* Adjoint contraction for parameter gradients (no counterpart in the
  * reference): for every chain c, out[2c] = lam_c . mu_c and

@@ -35,6 +35,8 @@ class Problem(C.Structure):
         ("B_col", C.c_void_p),
         ("B_val", C.c_void_p),
         ("sigma", C.c_void_p),
+        ("diffusion", C.c_void_p),
+        ("kappa_sq_field", C.c_void_p),
     ]
 
 
@@ -84,6 +86,7 @@ _SIGS = {
     "mgmc_coarse": [_vp, _vp, _vp, C.c_int, _vp],
     "mgmc_normals": [_vp, C.c_uint32, _vp],
     "mgmc_level_stencil": [_vp, C.c_int, _vp, C.POINTER(C.c_int)],
+    "mgmc_level_stencils": [C.POINTER(Problem), C.c_int, C.c_int, _vp],
     "mgmc_measurement_vector": [C.c_int, _vp, _vp, C.c_double, C.c_int64, _vp, _vp, C.POINTER(C.c_int64)],
     "mgmc_fem_stencil": [C.c_int, _vp, C.c_double, _vp],
     "mgmc_measurement_vector_weighted": [C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_double, C.c_int64, _vp, _vp,

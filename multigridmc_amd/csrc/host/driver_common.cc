@@ -233,6 +233,7 @@ mgmc_problem Problem::view() const {
   p.dim = dim;
   for (int d = 0; d < 3; ++d) p.n[d] = n[d];
   p.kappa_sq = kappa_sq;
+  if (!kappa_field.empty()) p.kappa_sq_field = kappa_field.data();
   if (fem) {
     p.A_rowptr = A_rowptr.data();
     p.A_col = A_col.data();
@@ -292,9 +293,24 @@ Problem build_problem(const GeneralParameters& gp, const LatticeParameters& lp, 
   P.n[2] = P.dim == 3 ? lp.nz : 1;
   P.N = 1;
   for (int d = 0; d < P.dim; ++d) P.N *= P.n[d] - 1;
-  if (pp.correlationlength_model != "constant")
-    die("the periodic correlation length model (variable coefficients) is not supported by the GPU library");
-  P.kappa_sq = 1.0 / (pp.Lambda * pp.Lambda);
+  if (pp.correlationlength_model == "constant") {
+    P.kappa_sq = 1.0 / (pp.Lambda * pp.Lambda);
+  } else {
+    // PeriodicCorrelationLengthModel (src/linear_operator/correlationlength_model.hh:83-105):
+    // Lambda(x) = Lambda_1 + Lambda_2 prod_d cos(pi x_d), kappa^2 = 1 / Lambda^2 at every vertex
+    if (pp.pde_model != "shiftedlaplace_fd")
+      die("the periodic correlation length model is supported with the finite-difference prior only");
+    const double L1 = 0.5 * (pp.Lambda_max + pp.Lambda_min), L2 = 0.5 * (pp.Lambda_max - pp.Lambda_min);
+    P.kappa_field.resize((size_t)P.N);
+    for (int64_t r = 0; r < P.N; ++r) {
+      int e[3] = {1, 1, 1};
+      euclid(P, r, e);
+      double lam = L2;
+      for (int d = 0; d < P.dim; ++d) lam *= std::cos(M_PI * ((double)e[d] * (1. / (double)P.n[d])));
+      lam += L1;
+      P.kappa_field[(size_t)r] = 1. / (lam * lam);
+    }
+  }
   if (pp.pde_model == "squared_shiftedlaplace_fd")
     die("the squared shifted-Laplace prior is not supported by the GPU library");
   if (pp.pde_model == "shiftedlaplace_fem") {

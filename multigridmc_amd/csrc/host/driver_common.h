@@ -99,6 +99,7 @@ struct Problem {
   int n[3] = {1, 1, 1};
   int64_t N = 0;
   double kappa_sq = 0.0;
+  std::vector<double> kappa_field;  // per vertex (periodic correlation-length model), empty: constant
   bool fem = false;
   std::vector<int64_t> A_rowptr, A_col;  // FEM prior as CSR
   std::vector<double> A_val;

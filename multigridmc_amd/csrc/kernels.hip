@@ -362,6 +362,142 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
 }
 
 // ---------------------------------------------------------------------------
+// Variable coefficients: every vertex has its own stencil, stored as planes
+// of the level's padded box (vc[plane * vol + p]): 7 planes on star levels
+// (-z, -y, -x, centre, +x, +y, +z), 27 on box levels (stencil index order).
+// The coefficients are shared by all chains.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, bool STAR>
+__device__ __forceinline__ T apply_var(const T* __restrict__ xp, const T* __restrict__ vc, long long vol, long long p,
+                                       const DevGeom& g) {
+  const long long pl = (long long)g.px * g.py;
+  T acc = (T)0;
+  if constexpr (STAR) {
+    if (DIM == 3) acc += vc[p] * xp[-pl];
+    acc += vc[vol + p] * xp[-g.px];
+    acc += vc[2 * vol + p] * xp[-1];
+    acc += vc[3 * vol + p] * xp[0];
+    acc += vc[4 * vol + p] * xp[1];
+    acc += vc[5 * vol + p] * xp[g.px];
+    if (DIM == 3) acc += vc[6 * vol + p] * xp[pl];
+  } else {
+#pragma unroll
+    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int e = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+          acc += vc[e * vol + p] * xp[dz * pl + dy * g.px + dx];
+        }
+  }
+  return acc;
+}
+
+// Chains per thread of the variable-coefficient kernels: the vertex's
+// stencil is loaded once and applied to VAR_CPT chains.
+constexpr int VAR_CPT = 8;
+
+// One colour phase of an SOR sweep with per-vertex stencils (the update of
+// SORSmoother::apply_sparse, src/smoother/sor_smoother.cc:56-77, with the
+// Gibbs right-hand side of SORSampler::apply, src/sampler/sor_sampler.cc:42-48:
+// c = f + sqrt(D (2 - omega) / omega) xi with the vertex's own D). A thread
+// updates one vertex of the colour in VAR_CPT consecutive chains (grid:
+// vertices x chain groups); the Philox normal of a vertex is the one the
+// constant-stencil kernels draw for it.
+template <typename T, int DIM, bool STAR, int NOISE>
+__global__ void __launch_bounds__(256) k_sweep_var(T* __restrict__ x, const T* __restrict__ f,
+                                                   const T* __restrict__ ext, DevGeom g, const T* __restrict__ vc,
+                                                   long long vol, double omega, int colour, int K, NoiseKey nk) {
+  const int nj = g.n1 - 1, nrow = nj * (DIM == 3 ? g.n2 - 1 : 1);
+  const int half = g.n0 / 2 + 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nrow * half) return;
+  const int i2 = (int)(t % half);
+  const int row = (int)(t / half);
+  const int c0 = blockIdx.y * VAR_CPT;
+  const int j = row % nj + 1;
+  const int k = DIM == 3 ? row / nj + 1 : 0;
+  int ipar;
+  if (STAR) {
+    ipar = (colour + j + k) & 1;
+  } else {
+    if ((j & 1) != ((colour >> 1) & 1)) return;
+    if (DIM == 3 && (k & 1) != ((colour >> 2) & 1)) return;
+    ipar = colour & 1;
+  }
+  const int i = 2 * i2 + ipar;
+  if (i < 1 || i > g.n0 - 1) return;
+  const long long p = ((long long)k * g.py + j) * g.px + i;
+  // the vertex's stencil, once for all chains of the thread
+  const int np = STAR ? 7 : 27;
+  T cf[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e)
+    if (e < np && (DIM == 3 || !STAR || (e != 0 && e != 6)) && (DIM == 3 || STAR || (e >= 9 && e < 18)))
+      cf[e] = vc[e * vol + p];
+  const T D = STAR ? cf[3] : cf[13];
+  const T ns = NOISE != NOISE_NONE ? (T)__builtin_sqrt((double)D * (2.0 - omega) / omega) : (T)0;
+  const long long pl = (long long)g.px * g.py;
+  const int ngx = g.px >> 3;
+  const uint32_t slot = (uint32_t)((((long long)k * g.py + j) * ngx + (i >> 3)) << 1) | (uint32_t)(i & 1);
+  for (int q = 0; q < VAR_CPT; ++q) {
+    const int chain = c0 + q;
+    if (chain >= K) break;
+    T* xp = x + chain * g.cs + p;
+    T acc = (T)0;
+    if constexpr (STAR) {
+      if (DIM == 3) acc += cf[0] * xp[-pl];
+      acc += cf[1] * xp[-g.px];
+      acc += cf[2] * xp[-1];
+      acc += cf[3] * xp[0];
+      acc += cf[4] * xp[1];
+      acc += cf[5] * xp[g.px];
+      if (DIM == 3) acc += cf[6] * xp[pl];
+    } else {
+#pragma unroll
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx)
+            acc += cf[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] * xp[dz * pl + dy * g.px + dx];
+    }
+    T c = f[chain * g.fcs + p];
+    if (NOISE == NOISE_PHILOX) {
+      float xi[4];
+      normals4(slot, (uint32_t)chain, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+      c += ns * (T)xi[(i & 7) >> 1];
+    }
+    if (NOISE == NOISE_EXTERNAL) c += ns * ext[chain * g.cs + p];
+    *xp += (T)omega * (c - acc) / D;
+  }
+}
+
+// r = f - A x (f may be NULL: r = -A x) with per-vertex stencils, one thread
+// per interior vertex of every chain.
+template <typename T, int DIM, bool STAR>
+__global__ void __launch_bounds__(256) k_residual_var(const T* __restrict__ x, const T* __restrict__ f,
+                                                      T* __restrict__ r, DevGeom g, const T* __restrict__ vc,
+                                                      long long vol, int K) {
+  const int nj = g.n1 - 1, nrow = nj * (DIM == 3 ? g.n2 - 1 : 1);
+  const int ni = g.n0 - 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nrow * ni) return;
+  const int i = (int)(t % ni) + 1;
+  const int row = (int)(t / ni);
+  const int j = row % nj + 1;
+  const int k = DIM == 3 ? row / nj + 1 : 0;
+  const long long p = ((long long)k * g.py + j) * g.px + i;
+  const int c0 = blockIdx.y * VAR_CPT;
+  for (int q = 0; q < VAR_CPT && c0 + q < K; ++q) {
+    const int chain = c0 + q;
+    const T acc = apply_var<T, DIM, STAR>(x + chain * g.cs + p, vc, vol, p, g);
+    r[chain * g.cs + p] = (f ? f[chain * g.fcs + p] : (T)0) - acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One wavefront of a lexicographic SOR sweep: the reference's own visiting
 // order (SORSmoother::apply_sparse, src/smoother/sor_smoother.cc:56-77,
 // vertices in increasing index order forward, decreasing backward) run in
@@ -380,7 +516,8 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
 template <typename T, int DIM, int NOISE>
 __global__ void __launch_bounds__(256) k_sweep_lex(T* __restrict__ x, const T* __restrict__ f,
                                                    const T* __restrict__ ext, DevGeom g, Stencil st, int s, int ay,
-                                                   int az, int K, NoiseKey nk) {
+                                                   int az, int K, NoiseKey nk, const T* __restrict__ vc,
+                                                   long long vol, int nplanes) {
   const int nj = g.n1 - 1, nk3 = DIM == 3 ? g.n2 - 1 : 1;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nj * nk3 * K) return;
@@ -394,25 +531,48 @@ __global__ void __launch_bounds__(256) k_sweep_lex(T* __restrict__ x, const T* _
   T* xp = x + chain * g.cs + off;
   const long long pl = (long long)g.px * g.py;
   T res = (T)0;
+  double diag = st.diag;
+  if (vc) {  // per-vertex stencils: 7 planes (star) or 27
+    if (nplanes == 7) {
+      if (DIM == 3) res += vc[off] * xp[-pl];
+      res += vc[vol + off] * xp[-g.px];
+      res += vc[2 * vol + off] * xp[-1];
+      res += vc[3 * vol + off] * xp[0];
+      res += vc[4 * vol + off] * xp[1];
+      res += vc[5 * vol + off] * xp[g.px];
+      if (DIM == 3) res += vc[6 * vol + off] * xp[pl];
+      diag = (double)vc[3 * vol + off];
+    } else {
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int e = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+            res += vc[e * vol + off] * xp[dz * pl + dy * g.px + dx];
+          }
+      diag = (double)vc[13 * vol + off];
+    }
+  } else {
 #pragma unroll
-  for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
+      for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) {
-        const double a = st.c[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        if (a != 0.0) res += (T)a * xp[dz * pl + dy * g.px + dx];
-      }
+        for (int dx = -1; dx <= 1; ++dx) {
+          const double a = st.c[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+          if (a != 0.0) res += (T)a * xp[dz * pl + dy * g.px + dx];
+        }
+  }
   T c = f[chain * g.fcs + off];
   if (NOISE == NOISE_PHILOX) {
     float xi[4];
     const int ngx = g.px >> 3;
     const uint32_t c0 = (uint32_t)((((long long)k * g.py + j) * ngx + (i >> 3)) << 1) | (uint32_t)(i & 1);
     normals4(c0, (uint32_t)chain, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
-    c += (T)st.noise_scale * (T)xi[(i & 7) >> 1];
+    c += (T)(vc ? __builtin_sqrt(diag * (2.0 - st.omega) / st.omega) : st.noise_scale) * (T)xi[(i & 7) >> 1];
   }
-  if (NOISE == NOISE_EXTERNAL) c += (T)st.noise_scale * ext[chain * g.cs + off];
-  *xp += (T)st.omega * (c - res) / (T)st.diag;
+  if (NOISE == NOISE_EXTERNAL)
+    c += (T)(vc ? __builtin_sqrt(diag * (2.0 - st.omega) / st.omega) : st.noise_scale) * ext[chain * g.cs + off];
+  *xp += (T)st.omega * (c - res) / (T)diag;
 }
 
 // ---------------------------------------------------------------------------

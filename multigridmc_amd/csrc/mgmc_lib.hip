@@ -146,6 +146,11 @@ struct Level {
   DevRows W[2];
   DevBuf<double> Sinv[2];
   DevBuf<double> SinvLex[2];  // capacitance inverses for the lexicographic order (mgmc_set_ordering)
+  // per-vertex stencils (variable coefficients): nplanes planes of the padded box
+  bool variable = false;
+  int nplanes = 0;
+  long long vol = 0;
+  DevBuf<double> vco64;
 };
 
 // k_lr_dot: B columns are split into up to kMaxLrSplit blocks of about
@@ -257,6 +262,13 @@ struct Engine : EngineBase {
   std::vector<DevBuf<T>> coef;  // the level's 27 stencil coefficients in precision T
   Field<T> ext;  // external noise staging (test hooks)
   std::vector<Field<T>> lw;     // lexicographic order: the SMW correction T^{-1} B u
+  std::vector<DevBuf<T>> vco;   // per-vertex stencils in precision T (FP32 engines; FP64 reads Level::vco64)
+  bool variable(int l) const { return C.L[l]->variable; }
+  const T* vcoef(int l) const {
+    if (!C.L[l]->variable) return nullptr;
+    if constexpr (sizeof(T) == sizeof(double)) return reinterpret_cast<const T*>(C.L[l]->vco64.p);
+    else return vco[l].p;
+  }
 
   // x of level l is logically zero but not yet written (zero_x_lazy): the
   // first plane-parity sweep or the coarse sample absorbs it; every other
@@ -297,6 +309,16 @@ struct Engine : EngineBase {
       for (int e = 0; e < 27; ++e) c27[e] = (T)C.L[l]->st.c[e];
       coef[l].upload(c27, 27);
     }
+    vco.resize(C.nlevel);
+    if constexpr (sizeof(T) != sizeof(double)) {
+      for (int l = 0; l < C.nlevel; ++l) {
+        if (!C.L[l]->variable) continue;
+        const std::vector<double>& v = C.H.levels[l].vst;
+        std::vector<T> t(v.size());
+        for (size_t q = 0; q < v.size(); ++q) t[q] = (T)v[q];
+        vco[l].upload(t.data(), t.size());
+      }
+    }
     for (int l = 0; l < C.nlevel; ++l) {
       size_t n = (size_t)C.L[l]->g.cs * C.K;
       x[l].alloc(n, C.stream);
@@ -323,7 +345,7 @@ struct Engine : EngineBase {
   }
   bool fused_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (lex() || !C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || Lv.variable || !C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
     return rb_ty(l) >= std::min(4, Lv.g.n1 - 1);
   }
   RbShape rb_shape(int l) const {
@@ -411,7 +433,7 @@ struct Engine : EngineBase {
   }
   bool march_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (lex() || !C.tune_march || Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || Lv.variable || !C.tune_march || Lv.star || Lv.g.dim != 3) return false;
     const MarchShape m = march_shape(l);
     return m.threads <= 1024 && m.lds <= (size_t)C.max_lds;
   }
@@ -460,7 +482,7 @@ struct Engine : EngineBase {
 
   bool planes_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (lex() || !C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
+    if (lex() || Lv.variable || !C.tune_planes || Lv.star || Lv.g.dim != 3) return false;
     const DevGeom& g = Lv.g;
     // one workgroup per (plane, chain): worth it only with enough workgroups
     return 2 * plane_half_threads(g) <= 1024 && (size_t)g.py * g.px * sizeof(T) <= (size_t)C.max_lds &&
@@ -560,9 +582,34 @@ struct Engine : EngineBase {
       nrows = nj * nk;
     }
     if (nrows <= 0) return;
+    NoiseKey nk = key(tag);
+    if (Lv.variable) {
+      const long long vthreads = (long long)(g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1) * (g.n0 / 2 + 1);
+      const dim3 vgrid(nblocks(vthreads), (C.K + VAR_CPT - 1) / VAR_CPT);
+      const bool vtimed = prof_begin(l);
+#define MGMC_VSWEEP(DIM, STAR, NOISE)                                                                               \
+  launch(vgrid, 256, 0, C.stream, k_sweep_var<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, geo(l), vcoef(l), Lv.vol, \
+         Lv.st.omega, colour, C.K, nk)
+#define MGMC_VSWEEP_N(DIM, STAR)                                  \
+  do {                                                            \
+    if (noise == NOISE_NONE) MGMC_VSWEEP(DIM, STAR, NOISE_NONE);  \
+    else if (noise == NOISE_PHILOX) MGMC_VSWEEP(DIM, STAR, NOISE_PHILOX); \
+    else MGMC_VSWEEP(DIM, STAR, NOISE_EXTERNAL);                  \
+  } while (0)
+      if (g.dim == 3) {
+        if (Lv.star) MGMC_VSWEEP_N(3, true);
+        else MGMC_VSWEEP_N(3, false);
+      } else {
+        if (Lv.star) MGMC_VSWEEP_N(2, true);
+        else MGMC_VSWEEP_N(2, false);
+      }
+#undef MGMC_VSWEEP_N
+#undef MGMC_VSWEEP
+      if (vtimed) prof_end();
+      return;
+    }
     long long threads = (long long)nrows * C.K * (g.px >> 3);
     dim3 grid(nblocks(threads)), blk(256);
-    NoiseKey nk = key(tag);
     const bool timed = prof_begin(l);
 #define MGMC_SWEEP(DIM, STAR, NOISE) \
   launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, geo(l), Lv.st, colour, nrows, C.K, \
@@ -653,7 +700,8 @@ struct Engine : EngineBase {
     for (int q = 0; q <= smax - smin; ++q) {
       const int s = forward ? smin + q : smax - q;
 #define MGMC_LEX(DIM, NOISE) \
-  launch(nblocks(threads), 256, 0, C.stream, k_sweep_lex<T, DIM, NOISE>, xp, fp, extp, gg, Lv.st, s, ay, az, C.K, nk)
+  launch(nblocks(threads), 256, 0, C.stream, k_sweep_lex<T, DIM, NOISE>, xp, fp, extp, gg, Lv.st, s, ay, az, C.K, nk, \
+         vcoef(l), Lv.vol, Lv.nplanes)
       if (g.dim == 3) {
         if (noise == NOISE_NONE) MGMC_LEX(3, NOISE_NONE);
         else if (noise == NOISE_PHILOX) MGMC_LEX(3, NOISE_PHILOX);
@@ -761,7 +809,7 @@ struct Engine : EngineBase {
   // fused residual + restriction on 3D star levels (k_residual_restrict_star3)
   bool rr_ok(int l) const {
     const Level& Lv = *C.L[l];
-    if (!C.tune_fused_rr || !Lv.star || Lv.g.dim != 3) return false;
+    if (Lv.variable || !C.tune_fused_rr || !Lv.star || Lv.g.dim != 3) return false;
     return rr_tyc(l) >= 1;
   }
   int rr_tyc(int l) const {
@@ -796,6 +844,22 @@ struct Engine : EngineBase {
            SZc, ntile, nseg);
   }
 
+  // r = f - A x on a variable level (precision U: the cycle's, or FP64 for the loop solver)
+  template <typename U>
+  void residual_var(int l, const U* xp, const U* fp, U* rp, const DevGeom& gg, const U* vc) {
+    const Level& Lv = *C.L[l];
+    const DevGeom& g = Lv.g;
+    const long long n = (long long)(g.n0 - 1) * (g.n1 - 1) * (g.dim == 3 ? g.n2 - 1 : 1);
+    const dim3 grid(nblocks(n), (C.K + VAR_CPT - 1) / VAR_CPT);
+    if (g.dim == 3) {
+      if (Lv.star) launch(grid, 256, 0, C.stream, k_residual_var<U, 3, true>, xp, fp, rp, gg, vc, Lv.vol, C.K);
+      else launch(grid, 256, 0, C.stream, k_residual_var<U, 3, false>, xp, fp, rp, gg, vc, Lv.vol, C.K);
+    } else {
+      if (Lv.star) launch(grid, 256, 0, C.stream, k_residual_var<U, 2, true>, xp, fp, rp, gg, vc, Lv.vol, C.K);
+      else launch(grid, 256, 0, C.stream, k_residual_var<U, 2, false>, xp, fp, rp, gg, vc, Lv.vol, C.K);
+    }
+  }
+
   // f_{l+1} = R (f_l - A_l x_l) - B_{l+1} Sigma^{-1} B_l^T x_l
   void residual_restrict(int l) override {
     Level& F = *C.L[l];
@@ -809,7 +873,10 @@ struct Engine : EngineBase {
       return;
     }
     dim3 gr = row_grid(F.g, F.g.px >> 3), gq = row_grid(Cc.g, (Cc.g.n0 + 3) / 4);
-    if (F.g.dim == 3) {
+    if (F.variable) {
+      residual_var<T>(l, X(l), f[l].p, r[l].p, geo(l), vcoef(l));
+      launch(gq, 256, 0, C.stream, F.g.dim == 3 ? k_restrict<T, 3> : k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
+    } else if (F.g.dim == 3) {
       if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
       else launch(gr, 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
       launch(gq, 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g);
@@ -1032,7 +1099,9 @@ struct Engine : EngineBase {
   void fine_residual_fp64(const double* x64, const double* b64, double* r64) override {
     Level& F = *C.L[0];
     dim3 gr = row_grid(F.g, F.g.px >> 3);
-    if (F.g.dim == 3) {
+    if (F.variable) {
+      residual_var<double>(0, x64, b64, r64, F.g, F.vco64.p);
+    } else if (F.g.dim == 3) {
       if (F.star) launch(gr, 256, 0, C.stream, k_residual<double, 3, true>, x64, b64, r64, F.g, F.st);
       else launch(gr, 256, 0, C.stream, k_residual<double, 3, false>, x64, b64, r64, F.g, F.st);
     } else {
@@ -1141,8 +1210,9 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
   C.nlevel = par->nlevel;
   C.k0 = (uint32_t)par->seed;
   C.k1 = (uint32_t)(par->seed >> 32);
-  HostProblem hp{prob->dim, {prob->n[0], prob->n[1], prob->n[2]}, prob->kappa_sq, prob->A_rowptr, prob->A_col,
-                 prob->A_val, prob->m, prob->B_rowptr, prob->B_col, prob->B_val, prob->sigma};
+  HostProblem hp{prob->dim,    {prob->n[0], prob->n[1], prob->n[2]}, prob->kappa_sq, prob->A_rowptr, prob->A_col,
+                 prob->A_val,  prob->m,       prob->B_rowptr, prob->B_col,  prob->B_val,    prob->sigma,
+                 prob->diffusion, prob->kappa_sq_field};
   C.H = build_hierarchy(hp, par->nlevel, par->omega, par->coarse_solver == MGMC_COARSE_CHOLESKY);
   if (par->stream) C.stream = (hipStream_t)par->stream;
   else {
@@ -1169,6 +1239,12 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
     Lv->star = HL.star;
     Lv->ncolour = HL.ncolour;
     Lv->N = HL.g.N;
+    if (HL.variable) {
+      Lv->variable = true;
+      Lv->nplanes = HL.nplanes();
+      Lv->vol = (long long)HL.g.vol;
+      Lv->vco64.upload(HL.vst.data(), HL.vst.size());
+    }
     if (C.m > 0) {
       Lv->bptr.upload(HL.bptr.data(), HL.bptr.size());
       {
@@ -1272,6 +1348,34 @@ int mgmc_measurement_vector_weighted(int dim, const int* n, const double* x0, do
   API_END
 }
 
+int mgmc_level_stencils(const mgmc_problem* prob, int nlevel, int level, double* out) {
+  API_BEGIN
+  if (!prob || !out) throw Error(MGMC_ERR_ARG, "null argument");
+  if (level < 0 || level >= nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  HostProblem hp{prob->dim,    {prob->n[0], prob->n[1], prob->n[2]}, prob->kappa_sq, prob->A_rowptr, prob->A_col,
+                 prob->A_val,  0,             nullptr,        nullptr,      nullptr,        nullptr,
+                 prob->diffusion, prob->kappa_sq_field};
+  const HostHierarchy H = build_hierarchy(hp, nlevel, 1.0, false);
+  const HostLevel& L = H.levels[level];
+  const Geom& g = L.g;
+  for (int64_t r = 0; r < g.N; ++r) {
+    int e[3];
+    g.euclid(r, e);
+    const int64_t p = g.padded(r);
+    for (int s = 0; s < 27; ++s) {
+      const int o[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+      bool inside = true;
+      for (int d = 0; d < 3; ++d) {
+        const int b = e[d] + o[d];
+        if (d < g.dim) inside = inside && b >= 1 && b <= g.n[d] - 1;
+        else inside = inside && o[d] == 0;
+      }
+      out[(size_t)r * 27 + s] = inside ? L.coef(p, s) : 0.0;
+    }
+  }
+  API_END
+}
+
 int mgmc_fem_stencil(int dim, const int* n, double kappa_sq, double* stencil) {
   API_BEGIN
   if (!n || !stencil) throw Error(MGMC_ERR_ARG, "null argument");
@@ -1283,6 +1387,8 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
   API_BEGIN
   if (!h || level < 0 || level >= h->C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
   const HostLevel& HL = h->C.H.levels[level];
+  if (HL.variable && stencil)
+    throw Error(MGMC_ERR_UNSUPPORTED, "the level has a stencil per vertex (mgmc_level_stencils gives them)");
   if (stencil) {
     if (HL.g.dim == 3) std::copy(HL.st, HL.st + 27, stencil);
     else std::copy(HL.st + 9, HL.st + 18, stencil);

@@ -96,6 +96,10 @@ void galerkin_stencil(int dim, const double* s, double* c) {
 }
 
 static void classify(HostLevel& L) {
+  if (L.variable) {  // star-ness is decided when the planes are built
+    L.ncolour = L.star ? 2 : (1 << L.g.dim);
+    return;
+  }
   L.star = true;
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
@@ -106,6 +110,195 @@ static void classify(HostLevel& L) {
   L.ncolour = L.star ? 2 : (1 << L.g.dim);
   L.diag = L.st[sidx(0, 0, 0)];
   if (!(L.diag > 0.0)) throw Error(MGMC_ERR_UNSUPPORTED, "stencil diagonal must be positive");
+}
+
+// ---- variable coefficients ----
+
+// Interior vertices of a level in reference order, with padded index.
+template <typename F>
+static void for_interior(const Geom& g, F&& f) {
+  const int nk = g.dim == 3 ? g.n[2] - 1 : 1;
+  for (int k = 1; k <= nk; ++k)
+    for (int j = 1; j <= g.n[1] - 1; ++j)
+      for (int i = 1; i <= g.n[0] - 1; ++i) {
+        const int kk = g.dim == 3 ? k : 0;
+        f(i, j, kk, ((int64_t)kk * g.py + j) * g.px + i);
+      }
+}
+
+static void check_positive_diagonal(const HostLevel& L) {
+  bool ok = true;
+  for_interior(L.g, [&](int, int, int, int64_t p) { ok = ok && L.coef(p, 13) > 0.0; });
+  if (!ok) throw Error(MGMC_ERR_UNSUPPORTED, "stencil diagonal must be positive");
+}
+
+// -div(a grad) + kappa^2 by finite differences on the vertices, scaled by the
+// cell volume as ShiftedLaplaceFDOperator (src/linear_operator/shiftedlaplace_fd_operator.cc:33-56):
+// the coupling of a vertex to its neighbour along axis d is -h^d a_e / h_d^2
+// with a_e the mean of a over the 2^(d-1) cells that share the edge, and the
+// diagonal h^d (kappa^2 + sum over the 2 d edges of a_e / h_d^2), edges to
+// boundary vertices included (Dirichlet elimination). For a = 1 and constant
+// kappa^2 this is the reference's operator, diagonal accumulated in the same order.
+static void fd_variable(HostLevel& L, double kappa_sq, const double* a_cell, const double* kappa_field) {
+  const Geom& g = L.g;
+  L.variable = true;
+  L.star = true;
+  L.vst.assign((size_t)kStarPlanes * g.vol, 0.0);
+  double hinv2[3] = {0, 0, 0}, cv = 1.0;
+  for (int d = 0; d < g.dim; ++d) {
+    const double h = 1. / double(g.n[d]);
+    hinv2[d] = 1. / (h * h);
+    cv *= h;
+  }
+  auto a_at = [&](int ci, int cj, int ck) -> double {  // cell (ci, cj, ck), 0-based
+    if (!a_cell) return 1.0;
+    return a_cell[((int64_t)(g.dim == 3 ? ck : 0) * g.n[1] + cj) * g.n[0] + ci];
+  };
+  // mean of a over the cells sharing the edge from vertex (i, j, k) to its + neighbour along d
+  auto edge = [&](int d, int i, int j, int k) -> double {
+    const int e[3] = {i, j, k};
+    int lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+      if (q == d) lo[q] = hi[q] = e[q];
+      else if (q < g.dim) {
+        lo[q] = e[q] - 1;
+        hi[q] = e[q];
+      } else lo[q] = hi[q] = 0;
+    }
+    double sum = 0.0;
+    int cnt = 0;
+    for (int ck = lo[2]; ck <= hi[2]; ++ck)
+      for (int cj = lo[1]; cj <= hi[1]; ++cj)
+        for (int ci = lo[0]; ci <= hi[0]; ++ci) {
+          sum += a_at(ci, cj, ck);
+          ++cnt;
+        }
+    return sum / cnt;
+  };
+  int64_t ref = 0;
+  for_interior(g, [&](int i, int j, int k, int64_t p) {
+    const double kap = kappa_field ? kappa_field[ref] : kappa_sq;
+    ++ref;
+    double diagonal = cv * kap;
+    const int e[3] = {i, j, k};
+    for (int d = 0; d < g.dim; ++d) {
+      int m[3] = {e[0], e[1], e[2]};
+      m[d] -= 1;
+      const double am = edge(d, m[0], m[1], m[2]), ap = edge(d, e[0], e[1], e[2]);
+      const int qm = 2 - d, qp = 4 + d;  // planes -x,-y,-z = 2,1,0; +x,+y,+z = 4,5,6
+      L.vst[(size_t)qm * g.vol + p] = -cv * hinv2[d] * am;
+      L.vst[(size_t)qp * g.vol + p] = -cv * hinv2[d] * ap;
+      diagonal += cv * hinv2[d] * (am + ap);
+    }
+    L.vst[(size_t)3 * g.vol + p] = diagonal;
+  });
+  // neighbours outside the interior are eliminated: their couplings multiply
+  // the zero boundary values, so they are kept (constant-stencil convention)
+}
+
+// A caller-supplied CSR matrix that is not one constant stencil: per-vertex
+// stencils, checked to couple nearest neighbours only and to be symmetric.
+static void stencils_from_csr(HostLevel& L, const int64_t* rp, const int64_t* ci, const double* v) {
+  const Geom& g = L.g;
+  std::vector<double> full((size_t)27 * g.vol, 0.0);
+  bool star = true;
+  double scale = 0.0;
+  for (int64_t r = 0; r < g.N; ++r) {
+    int a[3];
+    g.euclid(r, a);
+    const int64_t p = ((int64_t)a[2] * g.py + a[1]) * g.px + a[0];
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      int b[3];
+      g.euclid(ci[k], b);
+      const int o[3] = {b[0] - a[0], b[1] - a[1], g.dim == 3 ? b[2] - a[2] : 0};
+      for (int d = 0; d < 3; ++d)
+        if (o[d] < -1 || o[d] > 1) throw Error(MGMC_ERR_UNSUPPORTED, "A couples vertices that are not neighbours");
+      full[(size_t)sidx(o[0], o[1], o[2]) * g.vol + p] += v[k];
+      if (v[k] != 0.0 && (o[0] != 0) + (o[1] != 0) + (o[2] != 0) > 1) star = false;
+      scale = std::max(scale, std::fabs(v[k]));
+    }
+  }
+  const double tol = 1e-11 * scale;
+  const int64_t pl = (int64_t)g.px * g.py;
+  for_interior(g, [&](int, int, int, int64_t p) {
+    for (int e = 0; e < 27; ++e) {
+      const int dz = e / 9 - 1, dy = (e / 3) % 3 - 1, dx = e % 3 - 1;
+      if (g.dim == 2 && dz != 0) continue;
+      const int64_t q = p + dz * pl + dy * g.px + dx;
+      if (q < 0 || q >= g.vol) continue;
+      const double back = full[(size_t)(26 - e) * g.vol + q];  // entry (q, p)
+      if (back != 0.0 || full[(size_t)e * g.vol + p] != 0.0)
+        if (std::fabs(full[(size_t)e * g.vol + p] - back) > tol) throw Error(MGMC_ERR_UNSUPPORTED, "A is not symmetric");
+    }
+  });
+  L.variable = true;
+  L.star = star;
+  if (star) {
+    L.vst.assign((size_t)kStarPlanes * g.vol, 0.0);
+    for (int q = 0; q < kStarPlanes; ++q)
+      std::copy(full.begin() + (size_t)kStarIndex[q] * g.vol, full.begin() + (size_t)(kStarIndex[q] + 1) * g.vol,
+                L.vst.begin() + (size_t)q * g.vol);
+  } else {
+    L.vst.swap(full);
+  }
+}
+
+// Galerkin coarse stencils R A R^T of a variable level (R = P^T, P the tensor
+// product of {1/2, 1, 1/2}; src/linear_operator/linear_operator.cc:10-23):
+// for coarse vertex I, fine vertex i = 2I + s (weight p(s)) and its coupling
+// o to j = i + o, j interpolates to the coarse vertices J within one step
+// (weight p(j - 2J)), so A_c(I, J) = sum_s sum_o sum_J p(s) A(i, o) p(j - 2J).
+static void galerkin_variable(const HostLevel& F, HostLevel& Cc) {
+  const Geom& gf = F.g;
+  const Geom& gc = Cc.g;
+  Cc.variable = true;
+  Cc.star = false;
+  Cc.vst.assign((size_t)27 * gc.vol, 0.0);
+  const int zr = gf.dim == 3 ? 1 : 0;
+  const double w1[3] = {0.5, 1.0, 0.5};
+  for_interior(gc, [&](int I, int J, int K, int64_t pc) {
+    const int C3[3] = {I, J, K};
+    for (int sz = -zr; sz <= zr; ++sz)
+      for (int sy = -1; sy <= 1; ++sy)
+        for (int sx = -1; sx <= 1; ++sx) {
+          const int f3[3] = {2 * I + sx, 2 * J + sy, gf.dim == 3 ? 2 * K + sz : 0};
+          const double ps = w1[sx + 1] * w1[sy + 1] * (gf.dim == 3 ? w1[sz + 1] : 1.0);
+          const int64_t pf = ((int64_t)f3[2] * gf.py + f3[1]) * gf.px + f3[0];
+          for (int oz = -zr; oz <= zr; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+              for (int ox = -1; ox <= 1; ++ox) {
+                const double a = F.coef(pf, sidx(ox, oy, oz));
+                if (a == 0.0) continue;
+                const int j3[3] = {f3[0] + ox, f3[1] + oy, f3[2] + oz};
+                // coarse vertices J that j interpolates to, per direction
+                int Jl[3][2], nJ[3];
+                double wJ[3][2];
+                for (int d = 0; d < 3; ++d) {
+                  if (d >= gf.dim) {
+                    Jl[d][0] = 0;
+                    wJ[d][0] = 1.0;
+                    nJ[d] = 1;
+                  } else if (j3[d] % 2 == 0) {
+                    Jl[d][0] = j3[d] / 2;
+                    wJ[d][0] = 1.0;
+                    nJ[d] = 1;
+                  } else {
+                    Jl[d][0] = (j3[d] - 1) / 2;
+                    Jl[d][1] = (j3[d] + 1) / 2;
+                    wJ[d][0] = wJ[d][1] = 0.5;
+                    nJ[d] = 2;
+                  }
+                }
+                for (int a2 = 0; a2 < nJ[2]; ++a2)
+                  for (int a1 = 0; a1 < nJ[1]; ++a1)
+                    for (int a0 = 0; a0 < nJ[0]; ++a0) {
+                      const int D[3] = {Jl[0][a0] - C3[0], Jl[1][a1] - C3[1], gf.dim == 3 ? Jl[2][a2] - C3[2] : 0};
+                      const double w = wJ[0][a0] * wJ[1][a1] * wJ[2][a2];
+                      Cc.vst[(size_t)sidx(D[0], D[1], D[2]) * gc.vol + pc] += ps * a * w;
+                    }
+              }
+        }
+  });
 }
 
 // Fine stencil from ShiftedLaplaceFDOperator with constant kappa^2
@@ -131,8 +324,10 @@ static void fd_stencil(const Geom& g, double kappa_sq, double* st) {
 }
 
 // Recognise a caller-supplied CSR matrix as a constant stencil with
-// Dirichlet elimination; anything else is reported as unsupported.
-static void stencil_from_csr(const Geom& g, const int64_t* rp, const int64_t* ci, const double* v, double* st) {
+// Dirichlet elimination; returns false when the rows differ (the matrix is
+// then taken as per-vertex stencils). Couplings beyond nearest neighbours
+// are unsupported.
+static bool stencil_from_csr(const Geom& g, const int64_t* rp, const int64_t* ci, const double* v, double* st) {
   for (int i = 0; i < 27; ++i) st[i] = 0.0;
   int zr = g.dim == 3 ? 1 : 0;
   // first pass: a row whose whole 3^d neighbourhood is interior defines the stencil
@@ -186,8 +381,7 @@ static void stencil_from_csr(const Geom& g, const int64_t* rp, const int64_t* ci
           bool inside = b[0] >= 1 && b[0] <= g.n[0] - 1 && b[1] >= 1 && b[1] <= g.n[1] - 1;
           if (g.dim == 3) inside = inside && b[2] >= 1 && b[2] <= g.n[2] - 1;
           double want = inside ? st[s] : 0.0;
-          if (std::fabs(seen[s] - want) > tol || (hit[s] && !inside))
-            throw Error(MGMC_ERR_UNSUPPORTED, "A is not a constant-coefficient stencil (row " + std::to_string(r) + ")");
+          if (std::fabs(seen[s] - want) > tol || (hit[s] && !inside)) return false;
         }
   }
   for (int s = 0; s < 27; ++s) {
@@ -195,6 +389,7 @@ static void stencil_from_csr(const Geom& g, const int64_t* rp, const int64_t* ci
     int t = sidx(-dx, -dy, -dz);
     if (std::fabs(st[s] - st[t]) > tol) throw Error(MGMC_ERR_UNSUPPORTED, "A is not symmetric");
   }
+  return true;
 }
 
 static int colour_of(const HostLevel& L, int i, int j, int k) {
@@ -348,14 +543,15 @@ static void compute_lowrank(HostLevel& L, const std::vector<Column>& cols, const
             for (int i = b0[0]; i <= b1[0]; ++i) {
               if (colour_of(L, i, j, kk) != col) continue;
               double acc = bb[bi(i, j, kk)];
+              const int64_t pv = ((int64_t)kk * g.py + j) * g.px + i;
               for (int dz = -zr; dz <= zr; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                   for (int dx = -1; dx <= 1; ++dx) {
                     if (dx == 0 && dy == 0 && dz == 0) continue;
-                    double a = L.st[sidx(dx, dy, dz)];
+                    double a = L.coef(pv, sidx(dx, dy, dz));
                     if (a != 0.0) acc -= a * w[bi(i + dx, j + dy, kk + dz)];
                   }
-              w[bi(i, j, kk)] = acc / (L.diag / omega);
+              w[bi(i, j, kk)] = acc / (L.coef(pv, 13) / omega);
             }
       }
       for (int kk = b0[2]; kk <= b1[2]; ++kk)
@@ -408,16 +604,14 @@ void lex_capacitance(const HostLevel& L, double omega, const std::vector<double>
   // padding after a chain's box is on the device)
   const size_t len = (size_t)g.vol + (size_t)pl + (size_t)g.px + 1;
   std::vector<double> b(len), w(len);
-  const double dw = L.diag / omega;
   for (int dir = 0; dir < 2; ++dir) {
-    // offsets of the neighbours visited before a vertex, increasing index order
-    std::vector<std::pair<int64_t, double>> before;
+    // stencil entries of the neighbours visited before a vertex, increasing index order
+    std::vector<std::pair<int64_t, int>> before;
     for (int dz = -zr; dz <= zr; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-          const double a = L.st[sidx(dx, dy, dz)];
           const bool earlier = dz < 0 || (dz == 0 && (dy < 0 || (dy == 0 && dx < 0)));
-          if (a != 0.0 && (dx || dy || dz) && earlier == (dir == 1)) before.push_back({dz * pl + dy * g.px + dx, a});
+          if ((dx || dy || dz) && earlier == (dir == 1)) before.push_back({dz * pl + dy * g.px + dx, sidx(dx, dy, dz)});
         }
     std::vector<double> S((size_t)m * m, 0.0);
     for (int l = 0; l < m; ++l) {
@@ -429,8 +623,11 @@ void lex_capacitance(const HostLevel& L, double omega, const std::vector<double>
         const int64_t v = dir == 1 ? q : nvtx - 1 - q;
         const int64_t p = g.padded(v);
         double acc = b[p];
-        for (auto& o : before) acc -= o.second * w[p + o.first];
-        w[p] = acc / dw;
+        for (auto& o : before) {
+          const double a = L.coef(p, o.second);
+          if (a != 0.0) acc -= a * w[p + o.first];
+        }
+        w[p] = acc / (L.coef(p, 13) / omega);
       }
       for (int k = 0; k < m; ++k)
         for (int32_t e = L.bptr[k]; e < L.bptr[k + 1]; ++e) S[(size_t)k * m + l] += L.bval[e] * w[L.bidx[e]];
@@ -464,7 +661,7 @@ static void build_coarse(const HostLevel& L, const std::vector<Column>& cols, co
           if (b[0] < 1 || b[0] > g.n[0] - 1 || b[1] < 1 || b[1] > g.n[1] - 1) continue;
           if (g.dim == 3 && (b[2] < 1 || b[2] > g.n[2] - 1)) continue;
           int64_t p = ((int64_t)b[2] * g.py + b[1]) * g.px + b[0];
-          Q[(size_t)r * Nc + ref[p]] += L.st[sidx(dx, dy, dz)];
+          Q[(size_t)r * Nc + ref[p]] += L.coef(C.pidx[r], sidx(dx, dy, dz));
         }
   }
   for (size_t k = 0; k < cols.size(); ++k)
@@ -516,12 +713,24 @@ HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bo
   Geom g = Geom::make(p.dim, p.n);
   HostLevel L0;
   L0.g = g;
-  if (p.A_rowptr) stencil_from_csr(g, p.A_rowptr, p.A_col, p.A_val, L0.st);
-  else {
+  if (p.A_rowptr) {
+    if (p.diffusion || p.kappa_sq_field) throw Error(MGMC_ERR_ARG, "give A either as CSR or by its coefficient fields");
+    if (!stencil_from_csr(g, p.A_rowptr, p.A_col, p.A_val, L0.st))
+      stencils_from_csr(L0, p.A_rowptr, p.A_col, p.A_val);
+  } else if (p.diffusion || p.kappa_sq_field) {
+    if (!p.kappa_sq_field && !(p.kappa_sq >= 0.0)) throw Error(MGMC_ERR_ARG, "kappa^2 must be >= 0");
+    const int64_t ncell = (int64_t)g.n[0] * g.n[1] * (g.dim == 3 ? g.n[2] : 1);
+    for (int64_t c = 0; p.diffusion && c < ncell; ++c)
+      if (!(p.diffusion[c] > 0.0)) throw Error(MGMC_ERR_ARG, "the diffusion coefficient must be positive");
+    for (int64_t v = 0; p.kappa_sq_field && v < g.N; ++v)
+      if (!(p.kappa_sq_field[v] >= 0.0)) throw Error(MGMC_ERR_ARG, "kappa^2 must be >= 0");
+    fd_variable(L0, p.kappa_sq, p.diffusion, p.kappa_sq_field);
+  } else {
     if (!(p.kappa_sq >= 0.0)) throw Error(MGMC_ERR_ARG, "kappa^2 must be >= 0");
     fd_stencil(g, p.kappa_sq, L0.st);
   }
   classify(L0);
+  if (L0.variable) check_positive_diagonal(L0);
   std::vector<Column> cols(p.m);
   if (p.m > 0) {
     if (!p.B_rowptr) throw Error(MGMC_ERR_ARG, "m > 0 needs B");
@@ -537,8 +746,10 @@ HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bo
     HostLevel& F = H.levels.back();
     HostLevel C;
     C.g = F.g.coarse();
-    galerkin_stencil(C.g.dim, F.st, C.st);
+    if (F.variable) galerkin_variable(F, C);
+    else galerkin_stencil(C.g.dim, F.st, C.st);
     classify(C);
+    if (C.variable) check_positive_diagonal(C);
     H.levels.push_back(C);
   }
   for (int l = 0; l < nlevel; ++l) {

@@ -53,12 +53,32 @@ struct SparseRows {
   int nrows() const { return (int)row.size(); }
 };
 
+// Offsets of the stencil planes of a variable star level, in stencil-index
+// order: -z, -y, -x, centre, +x, +y, +z.
+constexpr int kStarPlanes = 7;
+constexpr int kStarIndex[kStarPlanes] = {4, 10, 12, 13, 14, 16, 22};
+
 struct HostLevel {
   Geom g;
   bool star = true;      // 5/7-point stencil -> red-black; else 2^d colours
   int ncolour = 2;
   double st[27] = {0};   // stencil, index (dz+1)*9 + (dy+1)*3 + (dx+1)
   double diag = 0;
+  // Variable coefficients: a stencil per vertex, stored as planes of vol
+  // values in the padded layout (vst[plane * vol + p]); 7 planes
+  // (kStarIndex) on star levels, 27 on box levels; zero at boundary slots.
+  // st / diag are unused then.
+  bool variable = false;
+  std::vector<double> vst;
+  int nplanes() const { return variable ? (star ? kStarPlanes : 27) : 0; }
+  // coefficient e (stencil index) of the vertex at padded index p
+  double coef(int64_t p, int e) const {
+    if (!variable) return st[e];
+    if (!star) return vst[(size_t)e * g.vol + p];
+    for (int q = 0; q < kStarPlanes; ++q)
+      if (kStarIndex[q] == e) return vst[(size_t)q * g.vol + p];
+    return 0.0;
+  }
   int m = 0;
   // columns of B_l as (padded index, value), used for z = B^T x
   std::vector<int32_t> bptr{0};
@@ -88,6 +108,8 @@ struct HostProblem {
   const int64_t* B_col;
   const double* B_val;
   const double* sigma;
+  const double* diffusion = nullptr;       // cell-wise a (prod n_d values, x fastest) or NULL
+  const double* kappa_sq_field = nullptr;  // vertex-wise kappa^2 (reference order) or NULL
 };
 
 struct HostHierarchy {

@@ -71,20 +71,40 @@ def Lattice3d(nx, ny, nz):
 
 
 class ShiftedLaplaceFDOperator:
-    """Prior precision h^d (-Delta_h + kappa^2) with Dirichlet elimination
-    (src/linear_operator/shiftedlaplace_fd_operator.cc:9-57), constant
-    kappa^2 = 1 / Lambda^2 (src/linear_operator/correlationlength_model.hh:45-66).
-    The HIP library assembles the stencil itself from (lattice, kappa_sq)."""
+    """Prior precision h^d (-div(a grad_h) + kappa^2) with Dirichlet elimination
+    (src/linear_operator/shiftedlaplace_fd_operator.cc:9-57): the reference's
+    operator for a = 1 and kappa^2 = 1 / Lambda^2 constant
+    (src/linear_operator/correlationlength_model.hh:45-66) or per vertex (its
+    correlation-length models, :83-105). The HIP library assembles the
+    stencils itself from (lattice, kappa_sq, fields)."""
 
     m_lowrank = 0
 
     def __init__(self, lattice, kappa_sq, diffusion=1.0):
-        """diffusion: a constant factor s on the Laplacian, h^d (-s Delta_h +
-        kappa^2) (1 in the reference; other values are passed to the library
-        as a CSR matrix)."""
+        """kappa_sq: a constant or one value per vertex (reference order).
+        diffusion: a constant factor s on the Laplacian, h^d (-s Delta_h +
+        kappa^2) (1 in the reference; other constants are passed to the
+        library as a CSR matrix), or one value of a per cell (cell (i, j, k)
+        at (k n_y + j) n_x + i): -div(a grad) with the coupling across an
+        edge the mean of a over the cells sharing it (passed to the library
+        as a field; per-vertex stencils on every level)."""
         self.lattice = lattice
-        self.kappa_sq = float(kappa_sq)
-        self.diffusion = float(diffusion)
+        self.kappa_field = None
+        self.diffusion_field = None
+        if np.ndim(kappa_sq) == 0:
+            self.kappa_sq = float(kappa_sq)
+        else:
+            self.kappa_field = np.ascontiguousarray(kappa_sq, dtype=np.float64).reshape(lattice.Nvertex)
+            self.kappa_sq = 0.0
+        if np.ndim(diffusion) == 0:
+            self.diffusion = float(diffusion)
+        else:
+            ncell = int(np.prod(lattice.n))
+            self.diffusion_field = np.ascontiguousarray(diffusion, dtype=np.float64).reshape(ncell)
+            self.diffusion = 1.0
+
+    def is_variable(self):
+        return self.kappa_field is not None or self.diffusion_field is not None
 
     def get_ndof(self):
         return self.lattice.Nvertex
@@ -118,7 +138,71 @@ class ShiftedLaplaceFDOperator:
             st[centre + strides[k]] -= cv * hinv2
         return st
 
+    def _edge_means(self, d):
+        """E[v] = mean of a over the cells sharing the edge from vertex v to
+        v + e_d, for v with v_d in 0..n_d-1 and the other components interior
+        (array indexed [k, j, i] in 3D, [j, i] in 2D), summed in the library's
+        order (cells with the slower components first)."""
+        L = self.lattice
+        dim = L.dim
+        a = self.diffusion_field.reshape(tuple(L.n[::-1]))  # [k, j, i]
+        ax = dim - 1 - d  # array axis of direction d
+        out = None
+        cnt = 0
+        # the other directions take cells v_q - 1 and v_q (q != d), in order z, y, x
+        shifts = [(0,) if q == ax else (0, 1) for q in range(dim)]
+        for combo in itertools.product(*shifts):
+            sl = tuple(slice(None) if q == ax else slice(combo[q], a.shape[q] - 1 + combo[q]) for q in range(dim))
+            term = a[sl]
+            out = term.copy() if out is None else out + term
+            cnt += 1
+        return out / cnt
+
+    def _variable_sparse(self):
+        L = self.lattice
+        dim = L.dim
+        N = L.Nvertex
+        cv = 1.0
+        hinv2 = []
+        for v in L.n:
+            h = 1.0 / v
+            hinv2.append(1.0 / (h * h))
+            cv *= h
+        kap = self.kappa_field if self.kappa_field is not None else np.full(N, self.kappa_sq)
+        diag = cv * kap
+        idx = L.vertex_euclidean(np.arange(N))  # 1-based (i, j, k)
+        rows, cols, vals = [], [], []
+        for d in range(dim):
+            if self.diffusion_field is None:
+                am = ap = np.ones(N)
+            else:
+                E = self._edge_means(d)
+                # E index: direction d runs over 0..n_d-1, the others over 1..n_q-1 (stored 0-based)
+                pos = [idx[:, q] - 1 for q in range(dim)]
+                pos_m = list(pos)
+                pos_p = list(pos)
+                pos_m[d] = idx[:, d] - 1
+                pos_p[d] = idx[:, d]
+                am = E[tuple(pos_m[::-1])]
+                ap = E[tuple(pos_p[::-1])]
+            diag = diag + cv * hinv2[d] * (am + ap)
+            for s, coef in ((-1, -cv * hinv2[d] * am), (1, -cv * hinv2[d] * ap)):
+                j = idx.copy()
+                j[:, d] += s
+                ok = (j[:, d] >= 1) & (j[:, d] <= L.n[d] - 1)
+                rows.append(np.arange(N)[ok])
+                cols.append(L.vertex_index(j[ok]))
+                vals.append(coef[ok])
+        rows.append(np.arange(N))
+        cols.append(np.arange(N))
+        vals.append(diag)
+        A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+        A.sort_indices()
+        return A
+
     def get_sparse(self):
+        if self.is_variable():
+            return self._variable_sparse()
         L = self.lattice
         N = L.Nvertex
         idx = L.vertex_euclidean(np.arange(N))
@@ -222,6 +306,37 @@ def V_sphere(radius, dim):
     if dim == 1:
         return 2.0 * radius
     return 2.0 * math.pi / dim * radius * radius * V_sphere(radius, dim - 2)
+
+
+def periodic_kappa_sq(lattice, Lambda_min, Lambda_max):
+    """kappa^2 = 1 / Lambda(x)^2 at every vertex for the reference's
+    PeriodicCorrelationLengthModel (src/linear_operator/correlationlength_model.hh:83-105):
+    Lambda(x) = Lambda_1 + Lambda_2 prod_d cos(pi x_d), Lambda_1,2 = (max +- min) / 2."""
+    L1 = 0.5 * (Lambda_max + Lambda_min)
+    L2 = 0.5 * (Lambda_max - Lambda_min)
+    X = lattice.vertex_coordinates(np.arange(lattice.Nvertex))
+    lam = np.full(lattice.Nvertex, L2)
+    for d in range(lattice.dim):
+        lam = lam * np.cos(math.pi * X[:, d])
+    lam = lam + L1
+    return 1.0 / (lam * lam)
+
+
+def block_diffusion(lattice, blocks, low, high, seed):
+    """A piecewise-constant diffusion coefficient: `blocks` (per direction)
+    equal blocks of cells with values drawn uniformly from [low, high] by
+    numpy's PCG64 with `seed`; one value per cell (cell (i, j, k) at
+    (k n_y + j) n_x + i)."""
+    dim = lattice.dim
+    blocks = [blocks] * dim if np.ndim(blocks) == 0 else list(blocks)
+    for d in range(dim):
+        if lattice.n[d] % blocks[d]:
+            raise ValueError("the blocks must divide the lattice")
+    vals = np.random.default_rng(seed).uniform(low, high, size=tuple(blocks[::-1]))
+    for d in range(dim):
+        ax = dim - 1 - d
+        vals = np.repeat(vals, lattice.n[d] // blocks[d], axis=ax)
+    return np.ascontiguousarray(vals.reshape(-1))
 
 
 def gaussian_ball_integral(width, dim):

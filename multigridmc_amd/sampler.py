@@ -45,7 +45,15 @@ def _problem(op):
     p.kappa_sq = op.kappa_sq
     keep = []
     prior = getattr(op, "base", op)
-    if not isinstance(prior, ShiftedLaplaceFDOperator) or prior.diffusion != 1.0:
+    if isinstance(prior, ShiftedLaplaceFDOperator) and prior.is_variable():
+        # coefficient fields: the library assembles per-vertex stencils
+        if prior.diffusion_field is not None:
+            keep.append(prior.diffusion_field)
+            p.diffusion = ptr(prior.diffusion_field)
+        if prior.kappa_field is not None:
+            keep.append(prior.kappa_field)
+            p.kappa_sq_field = ptr(prior.kappa_field)
+    elif not isinstance(prior, ShiftedLaplaceFDOperator) or prior.diffusion != 1.0:
         A = prior.get_sparse().tocsr()
         A.sort_indices()
         rp, ci, v = i64(A.indptr), i64(A.indices), f64(A.data)

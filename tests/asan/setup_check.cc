@@ -19,6 +19,7 @@ using namespace mgmc;
 struct Case {
   int dim, n[3], m, nlevel;
   bool fem;
+  bool variable = false;  // cell-wise diffusion and vertex-wise kappa^2 fields
 };
 
 static std::vector<double> run(const Case& c, double omega) {
@@ -68,12 +69,20 @@ static std::vector<double> run(const Case& c, double omega) {
       arp.push_back((int64_t)aci.size());
     }
   }
+  std::vector<double> a_cell, kap;
+  if (c.variable) {
+    const int64_t ncell = (int64_t)c.n[0] * c.n[1] * (c.dim == 3 ? c.n[2] : 1);
+    for (int64_t q = 0; q < ncell; ++q) a_cell.push_back(1.0 + 9.0 * U(rng));
+    for (int64_t q = 0; q < g.N; ++q) kap.push_back(5.0 + 10.0 * U(rng));
+  }
   HostProblem p{c.dim, {c.n[0], c.n[1], c.n[2]}, 10.0, c.fem ? arp.data() : nullptr, c.fem ? aci.data() : nullptr,
-                c.fem ? av.data() : nullptr, c.m, brp.data(), bci.data(), bv.data(), sigma.data()};
+                c.fem ? av.data() : nullptr, c.m, brp.data(), bci.data(), bv.data(), sigma.data(),
+                c.variable ? a_cell.data() : nullptr, c.variable ? kap.data() : nullptr};
   HostHierarchy H = build_hierarchy(p, c.nlevel, omega, true);
   std::vector<double> out;
   for (auto& L : H.levels) {
     out.insert(out.end(), L.st, L.st + 27);
+    out.insert(out.end(), L.vst.begin(), L.vst.end());
     for (int d = 0; d < 2; ++d) {
       out.insert(out.end(), L.Sinv[d].begin(), L.Sinv[d].end());
       out.insert(out.end(), L.W[d].val.begin(), L.W[d].val.end());
@@ -92,6 +101,7 @@ int main() {
   const Case cases[] = {
       {3, {16, 16, 16}, 3, 3, false}, {3, {12, 16, 8}, 3, 2, false}, {2, {16, 16, 1}, 2, 3, false},
       {2, {24, 20, 1}, 3, 2, false},  {3, {16, 16, 16}, 2, 3, true},  {2, {32, 32, 1}, 4, 3, true},
+      {3, {16, 16, 16}, 3, 3, false, true}, {3, {12, 16, 8}, 2, 2, false, true}, {2, {32, 24, 1}, 3, 3, false, true},
   };
   for (const Case& c : cases)
     for (double omega : {1.0, 0.8}) {

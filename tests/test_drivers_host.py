@@ -12,7 +12,7 @@ def test_parameters_are_read_and_echoed(tmp_path):
                  "    cycle = 1 ( V-cycle )", "  timeseries: number of samples         = 2000",
                  "  convergence test: number of samples   = 512", "    Lambda = 0.3",
                  "    number of measurement points = 4", "    radius of individual measurements = 0.05",
-                 "  gpu: chains = 1, convergence chains per batch = 256, precision = fp64, seed = 7"]:
+                 "  gpu: chains = 1, convergence chains per batch = 256, precision = fp64, ordering = multicolour, seed = 7"]:
         assert line in out.splitlines(), line
 
 
@@ -22,12 +22,14 @@ def test_libconfig_syntax_variants(tmp_path):
         'gpu : { nchains = 2, convergence_chains = 0x20; precision = "fp" "32"; seed = 12L; };')
     assert "0x20" in cfg
     rc, out = run_driver("driver_mgmc", tmp_path, cfg_text=cfg)
-    assert "  gpu: chains = 2, convergence chains per batch = 32, precision = fp32, seed = 12" in out
+    assert ("  gpu: chains = 2, convergence chains per batch = 32, precision = fp32, ordering = multicolour, "
+            "seed = 12") in out
 
 
 @pytest.mark.parametrize("old,new,message", [
     ('smoother = "SSOR";\n    coarse_solver', 'smoother = "Jacobi";\n    coarse_solver', "invalid multigrid smoother"),
-    ('correlationlengthmodel = "constant"', 'correlationlengthmodel = "periodic"', "variable coefficients"),
+    ('pdemodel = "shiftedlaplace_fd";\n    correlationlengthmodel = "constant"',
+     'pdemodel = "shiftedlaplace_fem";\n    correlationlengthmodel = "periodic"', "finite-difference prior only"),
     ("measure_global = false", "measure_global = true", "measure_global"),
     ('pdemodel = "shiftedlaplace_fd"', 'pdemodel = "squared_shiftedlaplace_fd"', "squared shifted-Laplace"),
     ("    nlevel = 3;\n", "", "cannot read group 'multigrid'"),
