@@ -294,6 +294,42 @@ int mgmc_get_tuning(const mgmc_handle* h, const char* key, int* value);
 int mgmc_profile(mgmc_handle* h, int enable);
 int mgmc_profile_result(mgmc_handle* h, double* avg_ms, int64_t* count);
 
+/* ---- Full-problem Cholesky samplers (host) ------------------------------
+ * SparseCholeskySampler, LowRankCholeskySampler and DenseCholeskySampler
+ * (src/sampler/cholesky_sampler.cc:9-129, cholesky_sampler.hh:52-86): exact
+ * independent samples x ~ N(Q^{-1} f, Q^{-1}) of the problem's posterior
+ * Q = A + B Sigma^{-1} B^T (or prior, m = 0), the baselines of the MGMC
+ * sampler. Host code only (no GPU is used): nested-dissection ordering and a
+ * supernodal multifrontal factorisation on `nthreads` host threads (0 = all
+ * hardware threads), chains solved in blocks on those threads.
+ * "sparse" factorises Q, "lowrank" factorises A and applies the reference's
+ * QR-based correction for B Sigma^{-1} B^T, "dense" factorises Q without
+ * reordering (at most 16384 unknowns). Replaces the constructors
+ * (cholesky_sampler.cc:9-23, :26-38, :41-129). */
+typedef struct mgmc_cholesky mgmc_cholesky;
+enum mgmc_factorisation { MGMC_FACTOR_SPARSE = 0, MGMC_FACTOR_LOWRANK = 1, MGMC_FACTOR_DENSE = 2 };
+int mgmc_cholesky_create(const mgmc_problem* problem, int factorisation, int nchains, uint64_t seed, int nthreads,
+                         mgmc_cholesky** out);
+int mgmc_cholesky_destroy(mgmc_cholesky* c);
+/* Unknowns, entries of the factor L (the reference's nnz(L) print-out,
+ * cholesky_wrapper.cc:85-91), supernodes and set-up seconds (any may be NULL). */
+int mgmc_cholesky_info(const mgmc_cholesky* c, int64_t* n, int64_t* nnz_L, int64_t* nsupernodes, double* seconds);
+/* Number of chains of the following mgmc_cholesky_sample calls (one
+ * factorisation serves any number; the reference deep-copies its sampler per
+ * thread instead, cholesky_sampler.hh:103-110). */
+int mgmc_cholesky_set_nchains(mgmc_cholesky* c, int nchains);
+/* CholeskySampler::fix_rhs / unfix_rhs (cholesky_sampler.hh:75-90). */
+int mgmc_cholesky_fix_rhs(mgmc_cholesky* c, const double* f);
+int mgmc_cholesky_unfix_rhs(mgmc_cholesky* c);
+/* CholeskySampler::apply (cholesky_sampler.hh:52-66) for every chain: x
+ * (nchains x N, reference order) receives independent samples. f: N values
+ * shared by all chains (f_per_chain = 0) or nchains x N; ignored after
+ * fix_rhs. xi: NULL draws the normals from Philox4x32-10 (one sample counter
+ * per call); else nchains x N given normals, consumed in the factor's order
+ * (a deterministic map xi -> x whose covariance is Q^{-1}; xi = 0 gives
+ * the mean Q^{-1} f). */
+int mgmc_cholesky_sample(mgmc_cholesky* c, const double* f, int f_per_chain, const double* xi, double* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,14 @@ _SIGS = {
     "mgmc_get_tuning": [_vp, C.c_char_p, C.POINTER(C.c_int)],
     "mgmc_profile": [_vp, C.c_int],
     "mgmc_profile_result": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+    "mgmc_cholesky_create": [C.POINTER(Problem), C.c_int, C.c_int, C.c_uint64, C.c_int, C.POINTER(_vp)],
+    "mgmc_cholesky_destroy": [_vp],
+    "mgmc_cholesky_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                           C.POINTER(C.c_double)],
+    "mgmc_cholesky_set_nchains": [_vp, C.c_int],
+    "mgmc_cholesky_fix_rhs": [_vp, _vp],
+    "mgmc_cholesky_unfix_rhs": [_vp],
+    "mgmc_cholesky_sample": [_vp, _vp, C.c_int, _vp, _vp],
 }
 
 _LIB = None

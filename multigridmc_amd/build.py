@@ -16,12 +16,16 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-W
 def build(verbose=True):
     obj = os.path.join(CSRC, "setup.o")
     oobj = os.path.join(CSRC, "operators.o")
+    cobj = os.path.join(CSRC, "cholesky_host.o")
     hobj = os.path.join(CSRC, "mgmc_lib.o")
     cmds = [
         ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "setup.cc"), "-o", obj],
         ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "operators.cc"), "-o", oobj],
+        # host Cholesky samplers: AVX2/FMA (every x86-64 server CPU since 2013)
+        ["g++", "-O3", "-mavx2", "-mfma", "-std=c++17", "-fPIC", "-Wall", "-Wno-unknown-pragmas", "-c", os.path.join(CSRC, "cholesky_host.cc"),
+         "-o", cobj],
         [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mgmc_lib.hip"), "-o", hobj],
-        [HIPCC, "--offload-arch=gfx950", "-shared", hobj, obj, oobj, "-o", LIB + ".tmp"],
+        [HIPCC, "--offload-arch=gfx950", "-shared", hobj, obj, oobj, cobj, "-lpthread", "-o", LIB + ".tmp"],
     ]
     for c in cmds:
         if verbose:

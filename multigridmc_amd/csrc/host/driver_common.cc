@@ -424,12 +424,98 @@ SamplerSpec ssor_sampler(const Problem& P, const SmootherParameters& sm, const G
   return SamplerSpec{&P, par};
 }
 
-SamplerSpec cholesky_sampler(const Problem& P, const GpuParameters& gpu) {
-  // the whole lattice as the coarsest level of a one-level hierarchy: the
-  // dense Cholesky sampler x = L^{-T}(xi + L^{-1} f)
-  mgmc_params par = base_params(gpu);
-  par.coarse_solver = MGMC_COARSE_CHOLESKY;
-  return SamplerSpec{&P, par};
+SamplerSpec cholesky_sampler(const Problem& P, const CholeskyParameters& ch, const GpuParameters& gpu) {
+  SamplerSpec s{&P, base_params(gpu)};
+  s.cholesky = ch.factorisation == "lowrank" ? MGMC_FACTOR_LOWRANK
+               : ch.factorisation == "dense" ? MGMC_FACTOR_DENSE
+                                             : MGMC_FACTOR_SPARSE;
+  mgmc_problem p = P.view();
+  mgmc_cholesky* c = nullptr;
+  auto t0 = std::chrono::high_resolution_clock::now();
+  check(mgmc_cholesky_create(&p, s.cholesky, 1, gpu.seed, 0, &c), "mgmc_cholesky_create");
+  auto t1 = std::chrono::high_resolution_clock::now();
+  s.factor = std::shared_ptr<mgmc_cholesky>(c, [](mgmc_cholesky* q) { mgmc_cholesky_destroy(q); });
+  int64_t nnz = 0;
+  check(mgmc_cholesky_info(c, nullptr, &nnz, nullptr, nullptr), "mgmc_cholesky_info");
+  std::cout << std::endl;
+  std::cout << "  matrix size = " << P.N << " x " << P.N << ", nnz(L) = " << nnz << std::endl;
+  std::cout << "time for Cholesky factorisation = " << std::chrono::duration<double>(t1 - t0).count() << " s"
+            << std::endl;
+  return s;
+}
+
+namespace {
+
+struct GpuChains : Chains {
+  Handle s;
+  const Problem& P;
+  GpuChains(const SamplerSpec& spec, int K) : s(spec, K), P(*spec.P) { nchains = K; }
+  void prepare(const Exact& ex, const Observation& b) override {
+    check(mgmc_set_rhs(s.h, ex.f.data(), 0), "mgmc_set_rhs");
+    std::vector<double> x0((size_t)P.N * nchains, 0.0);
+    check(mgmc_set_state(s.h, x0.data()), "mgmc_set_state");
+    int64_t rp[2] = {0, (int64_t)b.idx.size()};
+    check(mgmc_set_observables(s.h, 1, rp, b.idx.data(), b.val.data()), "mgmc_set_observables");
+  }
+  void run(int nsteps, double* obs) override {
+    if (nsteps > 0) check(mgmc_run(s.h, nsteps, obs, 1), "mgmc_run");
+  }
+  void moments(int nsamples, double* mean, double* var) override {
+    check(mgmc_moments(s.h, nsamples, mean, var), "mgmc_moments");
+  }
+};
+
+struct CholeskyChains : Chains {
+  std::shared_ptr<mgmc_cholesky> factor;
+  mgmc_cholesky* c = nullptr;
+  const Problem& P;
+  Observation b;
+  std::vector<double> x;
+  CholeskyChains(const SamplerSpec& spec, int K) : factor(spec.factor), c(spec.factor.get()), P(*spec.P) {
+    nchains = K;
+    x.resize((size_t)P.N * K);
+  }
+  void prepare(const Exact& ex, const Observation& obs) override {
+    check(mgmc_cholesky_set_nchains(c, nchains), "mgmc_cholesky_set_nchains");
+    check(mgmc_cholesky_fix_rhs(c, ex.f.data()), "mgmc_cholesky_fix_rhs");
+    b = obs;
+  }
+  void run(int nsteps, double* obs) override {
+    for (int k = 0; k < nsteps; ++k) {
+      check(mgmc_cholesky_sample(c, nullptr, 0, nullptr, x.data()), "mgmc_cholesky_sample");
+      if (!obs) continue;
+      for (int ch = 0; ch < nchains; ++ch) {
+        double z = 0.0;
+        for (size_t e = 0; e < b.idx.size(); ++e) z += b.val[e] * x[(size_t)ch * P.N + b.idx[e]];
+        obs[(size_t)k * nchains + ch] = z;
+      }
+    }
+  }
+  void moments(int nsamples, double* mean, double* var) override {
+    std::vector<double> s1((size_t)P.N, 0.0), s2((size_t)P.N, 0.0);
+    for (int k = 0; k < nsamples; ++k) {
+      check(mgmc_cholesky_sample(c, nullptr, 0, nullptr, x.data()), "mgmc_cholesky_sample");
+      for (int ch = 0; ch < nchains; ++ch)
+        for (int64_t i = 0; i < P.N; ++i) {
+          const double v = x[(size_t)ch * P.N + i];
+          s1[i] += v;
+          s2[i] += v * v;
+        }
+    }
+    const double n = (double)nsamples * nchains;
+    for (int64_t i = 0; i < P.N; ++i) {
+      const double m = s1[i] / n;
+      if (mean) mean[i] = m;
+      if (var) var[i] = s2[i] / n - m * m;
+    }
+  }
+};
+
+}  // namespace
+
+std::unique_ptr<Chains> make_chains(const SamplerSpec& s, int nchains) {
+  if (s.cholesky >= 0) return std::make_unique<CholeskyChains>(s, nchains);
+  return std::make_unique<GpuChains>(s, nchains);
 }
 
 // ---------------------------------------------------------------------------
@@ -477,23 +563,15 @@ Exact exact_statistics(const Problem& P, const MultigridParameters& mg, const Ob
 // harnesses (src/measurements/*.cc)
 // ---------------------------------------------------------------------------
 
-static void prepare(Handle& s, const Problem& P, const Exact& ex, const Observation& b) {
-  check(mgmc_set_rhs(s.h, ex.f.data(), 0), "mgmc_set_rhs");
-  std::vector<double> x0((size_t)P.N * s.nchains, 0.0);
-  check(mgmc_set_state(s.h, x0.data()), "mgmc_set_state");
-  int64_t rp[2] = {0, (int64_t)b.idx.size()};
-  check(mgmc_set_observables(s.h, 1, rp, b.idx.data(), b.val.data()), "mgmc_set_observables");
-}
-
 void measure_sampling_time(const SamplerSpec& spec, const Exact& ex, const Observation& b, const SamplingParameters& sp,
                            const GpuParameters& gpu, const std::string& label, const std::string& filename) {
-  Handle s(spec, gpu.nchains);
-  prepare(s, *spec.P, ex, b);
-  check(mgmc_run(s.h, sp.nwarmup, nullptr, 1), "mgmc_run");
-  const int K = s.nchains;
+  std::unique_ptr<Chains> s = make_chains(spec, gpu.nchains);
+  s->prepare(ex, b);
+  s->run(sp.nwarmup, nullptr);
+  const int K = s->nchains;
   std::vector<double> data((size_t)sp.nsamples * K);
   auto t0 = std::chrono::high_resolution_clock::now();
-  check(mgmc_run(s.h, sp.nsamples, data.data(), 1), "mgmc_run");
+  s->run(sp.nsamples, data.data());
   auto t1 = std::chrono::high_resolution_clock::now();
   double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / ((double)sp.nsamples * K);
   printf("  %12s time per sample = %12.4f ms\n", label.c_str(), ms);
@@ -522,10 +600,10 @@ static std::vector<double> series(const SamplerSpec& spec, const Exact& ex, cons
   int done = 0;
   while (done < nsamples) {
     int K = std::min(batch, nsamples - done);
-    Handle s(spec, K);
-    prepare(s, *spec.P, ex, b);
+    std::unique_ptr<Chains> s = make_chains(spec, K);
+    s->prepare(ex, b);
     std::vector<double> obs((size_t)std::max(nsteps, 1) * K);
-    if (nsteps > 0) check(mgmc_run(s.h, nsteps, obs.data(), 1), "mgmc_run");
+    s->run(nsteps, obs.data());
     for (int c = 0; c < K; ++c)
       for (int j = 1; j <= nsteps; ++j) z[(size_t)(done + c) * (nsteps + 1) + j] = obs[(size_t)(j - 1) * K + c];
     done += K;
@@ -608,12 +686,12 @@ void measure_mean_squared_error(const SamplerSpec& spec, const Exact& ex, const 
 void posterior_statistics(const SamplerSpec& spec, const Exact& ex, const SamplingParameters& sp,
                           const MeasurementParameters& mp, const GpuParameters& gpu) {
   const Problem& P = *spec.P;
-  Handle s(spec, gpu.nchains);
+  std::unique_ptr<Chains> s = make_chains(spec, gpu.nchains);
   Observation none;
-  prepare(s, P, ex, none);
-  check(mgmc_run(s.h, sp.nwarmup, nullptr, 1), "mgmc_run");
+  s->prepare(ex, none);
+  s->run(sp.nwarmup, nullptr);
   std::vector<double> mean((size_t)P.N), var((size_t)P.N);
-  check(mgmc_moments(s.h, std::max(sp.nsamples, 1), mean.data(), var.data()), "mgmc_moments");
+  s->moments(std::max(sp.nsamples, 1), mean.data(), var.data());
   write_vtk("posterior.vtk", P, {{"mean", &mean}, {"variance", &var}, {"mean_exact", &ex.x_post}});
   if (P.dim == 2) write_vtk_circle(mp.sample_location, mp.radius, "sample_location.vtk");
 }

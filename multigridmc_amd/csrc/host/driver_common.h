@@ -8,6 +8,7 @@
 #define MGMC_HOST_DRIVER_COMMON_H
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -147,10 +148,28 @@ Exact exact_statistics(const Problem& P, const MultigridParameters& mg, const Ob
 struct SamplerSpec {
   const Problem* P;
   mgmc_params par;
+  int cholesky = -1;  // mgmc_factorisation of the host Cholesky sampler; -1: a GPU handle
+  std::shared_ptr<mgmc_cholesky> factor;  // its factorisation, shared by every set of chains
 };
 SamplerSpec multigridmc_sampler(const Problem& P, const MultigridParameters& mg, const GpuParameters& gpu);
 SamplerSpec ssor_sampler(const Problem& P, const SmootherParameters& sm, const GpuParameters& gpu);
-SamplerSpec cholesky_sampler(const Problem& P, const GpuParameters& gpu);  // exact, N <= 2048
+// the reference's sparse / lowrank / dense Cholesky sampler (host code of the
+// library); factorises once and prints the time as src/driver_mgmc.cc:174-192
+SamplerSpec cholesky_sampler(const Problem& P, const CholeskyParameters& ch, const GpuParameters& gpu);
+
+// K chains of one sampler behind the harnesses: a GPU handle (MGMC, SSOR) or
+// the host Cholesky sampler, whose samples are independent.
+struct Chains {
+  int nchains = 1;
+  virtual ~Chains() = default;
+  // right-hand side Q x_post, zero states, observable b
+  virtual void prepare(const Exact& ex, const Observation& b) = 0;
+  // nsteps samples per chain; obs (nsteps x nchains, may be NULL) receives b.x
+  virtual void run(int nsteps, double* obs) = 0;
+  // pointwise mean and variance over nsamples steps of all chains
+  virtual void moments(int nsamples, double* mean, double* var) = 0;
+};
+std::unique_ptr<Chains> make_chains(const SamplerSpec& s, int nchains);
 
 // Harnesses (src/measurements/*.cc).
 void measure_sampling_time(const SamplerSpec& s, const Exact& ex, const Observation& b, const SamplingParameters& sp,

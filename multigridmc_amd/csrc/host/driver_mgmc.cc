@@ -64,12 +64,9 @@ int main(int argc, char* argv[]) {
     const char* suffix;
     SamplerSpec spec;
   };
-  const bool chol_ok = P.N <= 2048;
-  if (general.do_cholesky && !chol_ok)
-    std::cout << "NOTE: the GPU Cholesky sampler is dense and limited to 2048 unknowns; skipping Cholesky runs"
-              << std::endl;
   Run runs[3] = {
-      {general.do_cholesky && chol_ok, "**** Cholesky ****", "Cholesky", "cholesky", cholesky_sampler(P, gpu)},
+      {general.do_cholesky, "**** Cholesky ****", "Cholesky", "cholesky",
+       general.do_cholesky ? cholesky_sampler(P, cholesky, gpu) : SamplerSpec{&P, {}}},
       {general.do_ssor, "**** SSOR ****", "SSOR", "ssor", ssor_sampler(P, smoother, gpu)},
       {general.do_multigridmc, "**** Multigrid MC ****", "MGMC", "multigridmc", multigridmc_sampler(P, multigrid, gpu)},
   };

@@ -24,6 +24,8 @@ namespace mgmc {
 
 static thread_local std::string g_error;
 
+void set_last_error(const std::string& message) { g_error = message; }
+
 #define HIP_CHECK(expr)                                                                    \
   do {                                                                                     \
     hipError_t _e = (expr);                                                                \

@@ -136,6 +136,9 @@ constexpr int64_t kMaxCoarseKernel = 2048;
 // (the factor is then required); otherwise it is built when it fits.
 HostHierarchy build_hierarchy(const HostProblem& p, int nlevel, double omega, bool coarse_cholesky = true);
 
+// Message returned by mgmc_last_error() on this thread (mgmc_lib.hip).
+void set_last_error(const std::string& message);
+
 // (Sigma + B^T T^{-1} B)^{-1} of a level for the lexicographic visiting order
 // ([0] backward, [1] forward; m x m row-major), T = D/omega + the couplings to
 // the vertices visited before.
