@@ -99,3 +99,46 @@ class DenseCholeskySampler(CholeskySampler):
     """Dense factorisation of Q in the reference's vertex order
     (cholesky_sampler.cc:26-38); at most 16384 unknowns."""
     factorisation = "dense"
+
+
+def _time_samplers(argv=None):
+    """Time per sample of the Cholesky samplers on n^3 lattices: the prior
+    kappa^2 = 10 with seeded ball measurements (radius 0.05, variance 1e-4),
+    right-hand side fixed, the reference's "time per sample" for the
+    Cholesky column of its summary (python/plot_summary.py:31)."""
+    import argparse
+    import time
+
+    from .operators import Lattice, MeasuredOperator, ShiftedLaplaceFDOperator, seeded_ball_measurements
+
+    ap = argparse.ArgumentParser(prog="python -m multigridmc_amd.cholesky")
+    ap.add_argument("sizes", type=int, nargs="+", help="cells per direction (3D lattices)")
+    ap.add_argument("--factorisation", choices=sorted(_FACTORISATION), action="append")
+    ap.add_argument("--measurements", type=int, default=32)
+    ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
+    ap.add_argument("--samples", type=int, default=5)
+    a = ap.parse_args(argv)
+    classes = {c.factorisation: c for c in (SparseCholeskySampler, LowRankCholeskySampler, DenseCholeskySampler)}
+    for n in a.sizes:
+        lat = Lattice(n, n, n)
+        centres, _, var = seeded_ball_measurements(lat, a.measurements, 0.05, 1e-4, 3)
+        op = MeasuredOperator(ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
+        f = np.random.default_rng(1).standard_normal(lat.Nvertex)
+        for name in a.factorisation or ["sparse", "lowrank"]:
+            row = {}
+            for K in (1, 8):
+                s = classes[name](op, nchains=K, nthreads=a.threads)
+                s.fix_rhs(f)
+                s.sample()
+                t0 = time.perf_counter()
+                for _ in range(a.samples):
+                    s.sample()
+                row[K] = 1e3 * (time.perf_counter() - t0) / (a.samples * K)
+                setup, nnz = s.setup_seconds, s.nnz_L
+                s.close()
+            print(f"{n}^3 {name:8s} N = {lat.Nvertex:8d}  nnz(L) = {nnz / 1e6:7.1f} M  set-up {setup:7.2f} s  "
+                  f"ms/sample: 1 chain {row[1]:8.2f}, 8 chains {row[8]:8.2f}", flush=True)
+
+
+if __name__ == "__main__":
+    _time_samplers()

@@ -1188,29 +1188,41 @@ __global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T*
 // ---------------------------------------------------------------------------
 // FP64 workgroups are capped at 512 threads, which gives the compiler 256
 // VGPRs for the plane window instead of spilling it.
-template <typename T>
+// Chain-blocked FP32 workgroups (CB = 2) are capped at 512 threads for the
+// same reason: the plane windows of two chains need up to 256 VGPRs.
+template <typename T, int CB = 1>
 constexpr int rb3_max_threads() {
-  return sizeof(T) == 8 ? 512 : 1024;
+  return sizeof(T) == 8 || CB > 1 ? 512 : 1024;
 }
 
-template <typename T, int NOISE>
-__global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
+// Chain blocking (CB = 2, f shared by the chains: fcs = 0): a workgroup
+// sweeps two consecutive chains of its tile and segment. Each thread keeps
+// the plane window of both chains, f(k) is loaded once for both and one
+// barrier per step serves both updates; each chain has its own LDS planes.
+// Every chain gets the noise, arithmetic and update order of CB = 1. The
+// per-chain code is a macro over separately named arrays, which the compiler
+// keeps in VGPRs (arrays indexed by the chain went to scratch memory).
+template <typename T, int NOISE, int CB>
+__global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
                                                     NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
                                                     T alpha) {
+  static_assert(CB == 1 || CB == 2, "one or two chains per workgroup");
   MGMC_DYNAMIC_SHARED(char, lds_raw);
-  // LDS: [guard][A (k even)][guard][A (k odd)][guard][B (k even)][guard][B (k odd)][guard];
+  // LDS per chain: [guard][A (k even)][guard][A (k odd)][guard][B (k even)][guard][B (k odd)][guard];
   // the zero guards stand for the x neighbours beyond the first and last row
   constexpr int LG = 8;
   const size_t asz = (size_t)(TY + 4) * g.px + LG, bsz = (size_t)(TY + 2) * g.px + LG;
-  T* A0 = reinterpret_cast<T*>(lds_raw) + LG;
-  T* B0 = A0 + 2 * asz;
-  if (threadIdx.x < 5 * LG) {
-    const int e = threadIdx.x / LG, o = threadIdx.x % LG;
-    T* gptr = e < 3 ? A0 - LG + e * asz : B0 - LG + (e - 2) * bsz;
+  const size_t csz = 2 * asz + 2 * bsz + LG;  // one chain's LDS
+  for (int i = threadIdx.x; i < 5 * LG * CB; i += blockDim.x) {
+    const int cb = i / (5 * LG), t = i % (5 * LG);
+    const int e = t / LG, o = t % LG;
+    T* A0c = reinterpret_cast<T*>(lds_raw) + cb * csz + LG;
+    T* gptr = e < 3 ? A0c - LG + e * asz : A0c + 2 * asz - LG + (e - 2) * bsz;
     gptr[o] = (T)0;
   }
+  T* const A0 = reinterpret_cast<T*>(lds_raw) + LG;  // chain 0; chain 1 at A0 + csz
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int r = tid / ngx, gx = tid % ngx;
@@ -1218,7 +1230,7 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
   const int tile = b % ntile;
   b /= ntile;
   const int seg = b % nseg;
-  const int chain = b / nseg;
+  const int chain = (b / nseg) * CB;  // first chain of the workgroup
   const int j0 = 1 + tile * TY;
   const int j = j0 - 2 + r;
   const bool active = r < TY + 4;
@@ -1229,25 +1241,33 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
   const int ke = ks + SZ < g.n2 ? ks + SZ : g.n2;  // owned interior planes [ks, ke)
   const long long pl = (long long)g.px * g.py;
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+  const long long cbase1 = cbase + g.cs;  // second chain (CB = 2)
   const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
   const uint32_t sample = *nk.sample;
   const int second = 1 - first;
 
   T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
+  T xmm1[8], xm1[8], xc1[8], xp1[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = (T)0;
+  for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = xmm1[q] = xm1[q] = xc1[q] = xp1[q] = (T)0;
   // optional prolongate-add of the coarse correction while the planes are read
-  ProlongStream<T> pro;
-  pro.init(xcoarse && has_mem ? xcoarse + (long long)chain * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1,
-           alpha);
-  if (has_mem && ks - 2 >= 0) {
-    load8<T>(x + cbase + (ks - 2) * pl, xm);
-    pro.add(xm, ks - 2);
+  ProlongStream<T> pro, pro1;
+#define MGMC_RB3_INIT(XM, XC, PRO, CH, CBASE)                                                                    \
+  PRO.init(xcoarse && has_mem ? xcoarse + (long long)(CH) * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1, \
+           alpha);                                                                                                 \
+  if (has_mem && ks - 2 >= 0) {                                                                                    \
+    load8<T>(x + CBASE + (ks - 2) * pl, XM);                                                                       \
+    PRO.add(XM, ks - 2);                                                                                           \
+  }                                                                                                                \
+  if (has_mem) {                                                                                                   \
+    load8<T>(x + CBASE + (ks - 1) * pl, XC);                                                                       \
+    PRO.add(XC, ks - 1);                                                                                           \
   }
-  if (has_mem) {
-    load8<T>(x + cbase + (ks - 1) * pl, xc);
-    pro.add(xc, ks - 1);
+  MGMC_RB3_INIT(xm, xc, pro, chain, cbase)
+  if constexpr (CB > 1) {
+    MGMC_RB3_INIT(xm1, xc1, pro1, chain + 1, cbase1)
   }
+#undef MGMC_RB3_INIT
   // groups holding no interior vertex (the x padding) draw no noise and update nothing
   const bool inner = 8 * gx <= g.n0 - 1;
 
@@ -1264,66 +1284,92 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_sweep_rb3(const T* __r
     const bool storeB = owned && kb >= ks && kb <= ke - 1;
     const bool doB = storeB && inner;
     // loads first: next plane of x, f on plane k
-    if (has_mem && k + 1 <= g.n2) {
-      load8<T>(x + cbase + (long long)(k + 1) * pl, xp);
-      pro.add(xp, k + 1);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) xp[q] = (T)0;
+#define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
+  if (has_mem && k + 1 <= g.n2) {                         \
+    load8<T>(x + CBASE + (long long)(k + 1) * pl, XP);    \
+    PRO.add(XP, k + 1);                                   \
+  } else {                                                \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) XP[q] = (T)0; \
+  }
+    MGMC_RB3_LOAD(xp, pro, cbase)
+    if constexpr (CB > 1) {
+      MGMC_RB3_LOAD(xp1, pro1, cbase1)
     }
+#undef MGMC_RB3_LOAD
     if (doA) load8<T>(f + fbase + (long long)k * pl, fc);
     // noise of both colours while the loads are in flight
     const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
     float xiA[4] = {0.f, 0.f, 0.f, 0.f}, xiB[4] = {0.f, 0.f, 0.f, 0.f};
+    float xiA1[4] = {0.f, 0.f, 0.f, 0.f}, xiB1[4] = {0.f, 0.f, 0.f, 0.f};
     if (NOISE == NOISE_PHILOX) {
       const uint32_t cj = (uint32_t)(j * ngx + gx);
       const uint32_t cpl = (uint32_t)(g.py * ngx);
-      if (doA) normals4(((k * cpl + cj) << 1) | (uint32_t)iparA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
-      if (doB) normals4(((kb * cpl + cj) << 1) | (uint32_t)iparB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+      const uint32_t cA = ((k * cpl + cj) << 1) | (uint32_t)iparA, cB = ((kb * cpl + cj) << 1) | (uint32_t)iparB;
+      if (doA) normals4(cA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
+      if (doB) normals4(cB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+      if constexpr (CB > 1) {
+        if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiA1);
+        if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiB1);
+      }
     }
-    T* A = A0 + (k & 1) * asz;         // plane k before the first colour
-    const T* Bm = B0 + (kb & 1) * bsz;  // plane k-1 after the first colour (written last step)
-    T* Bk = B0 + (k & 1) * bsz;
-    if (active) LdsRow<T>::store(A + aoff, gx, ngx, xc);
+    if (active) LdsRow<T>::store(A0 + (k & 1) * asz + aoff, gx, ngx, xc);  // plane k before the first colour
+    if constexpr (CB > 1) {
+      if (active) LdsRow<T>::store(A0 + csz + (k & 1) * asz + aoff, gx, ngx, xc1);
+    }
     __syncthreads();
-    // first colour on plane k (owned rows and halo ring)
-    if (doA) {
-      const T* Arow = A + aoff;
-      T ym[8], yp[8];
-      LdsRow<T>::load(Arow - g.px, gx, ngx, ym);
-      LdsRow<T>::load(Arow + g.px, gx, ngx, yp);
-      const T left = gx > 0 ? LdsRow<T>::left(Arow, gx, ngx) : (T)0;
-      const T right = gx < ngx - 1 ? LdsRow<T>::right(Arow, gx, ngx) : (T)0;
-      const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)k * pl : nullptr;
-      if (iparA == 0)
-        star_update4<0, T, NOISE>(xc, left, right, ym, yp, xm, xp, fc, xiA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
-      else
-        star_update4<1, T, NOISE>(xc, left, right, ym, yp, xm, xp, fc, xiA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
+#define MGMC_RB3_UPDATE(XMM, XM, XC, XP, XIA, XIB, CBASE, AC)                                                      \
+  {                                                                                                                \
+    const T* A = (AC) + (k & 1) * asz;                                                                             \
+    const T* Bm = (AC) + 2 * asz + (kb & 1) * bsz; /* plane k-1 after the first colour (written last step) */     \
+    T* Bk = (AC) + 2 * asz + (k & 1) * bsz;                                                                        \
+    /* first colour on plane k (owned rows and halo ring) */                                                       \
+    if (doA) {                                                                                                     \
+      const T* Arow = A + aoff;                                                                                    \
+      T ym[8], yp[8];                                                                                              \
+      LdsRow<T>::load(Arow - g.px, gx, ngx, ym);                                                                   \
+      LdsRow<T>::load(Arow + g.px, gx, ngx, yp);                                                                   \
+      const T left = gx > 0 ? LdsRow<T>::left(Arow, gx, ngx) : (T)0;                                               \
+      const T right = gx < ngx - 1 ? LdsRow<T>::right(Arow, gx, ngx) : (T)0;                                       \
+      const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)k * pl : nullptr;                           \
+      if (iparA == 0)                                                                                              \
+        star_update4<0, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
+                                  ns);                                                                             \
+      else                                                                                                         \
+        star_update4<1, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
+                                  ns);                                                                             \
+    }                                                                                                              \
+    /* second colour on plane k-1 (owned rows, owned planes) */                                                    \
+    if (doB) {                                                                                                     \
+      const T* Brow = Bm + boff;                                                                                   \
+      T ym[8], yp[8];                                                                                              \
+      LdsRow<T>::load(Brow - g.px, gx, ngx, ym);                                                                   \
+      LdsRow<T>::load(Brow + g.px, gx, ngx, yp);                                                                   \
+      const T left = gx > 0 ? LdsRow<T>::left(Brow, gx, ngx) : (T)0;                                               \
+      const T right = gx < ngx - 1 ? LdsRow<T>::right(Brow, gx, ngx) : (T)0;                                       \
+      const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)kb * pl : nullptr;                          \
+      if (iparB == 0)                                                                                              \
+        star_update4<0, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
+                                  om_d, ns);                                                                       \
+      else                                                                                                         \
+        star_update4<1, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
+                                  om_d, ns);                                                                       \
+    }                                                                                                              \
+    if (storeB) store8<T>(xout + CBASE + (long long)kb * pl, XM);                                                  \
+    /* plane k after the first colour, for the second colour at the next step */                                   \
+    if (active && r >= 1 && r <= TY + 2) LdsRow<T>::store(Bk + boff, gx, ngx, XC);                                 \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                                                \
+      XMM[q] = XM[q];                                                                                              \
+      XM[q] = XC[q];                                                                                               \
+      XC[q] = XP[q];                                                                                               \
+    }                                                                                                              \
+  }
+    MGMC_RB3_UPDATE(xmm, xm, xc, xp, xiA, xiB, cbase, A0)
+    if constexpr (CB > 1) {
+      MGMC_RB3_UPDATE(xmm1, xm1, xc1, xp1, xiA1, xiB1, cbase1, A0 + csz)
     }
-    // second colour on plane k-1 (owned rows, owned planes)
-    if (doB) {
-      const T* Brow = Bm + boff;
-      T ym[8], yp[8];
-      LdsRow<T>::load(Brow - g.px, gx, ngx, ym);
-      LdsRow<T>::load(Brow + g.px, gx, ngx, yp);
-      const T left = gx > 0 ? LdsRow<T>::left(Brow, gx, ngx) : (T)0;
-      const T right = gx < ngx - 1 ? LdsRow<T>::right(Brow, gx, ngx) : (T)0;
-      const T* eg = NOISE == NOISE_EXTERNAL ? ext + cbase + (long long)kb * pl : nullptr;
-      if (iparB == 0)
-        star_update4<0, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
-      else
-        star_update4<1, T, NOISE>(xm, left, right, ym, yp, xmm, xc, fm, xiB, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, ns);
-    }
-    if (storeB) store8<T>(xout + cbase + (long long)kb * pl, xm);
-    // plane k after the first colour, for the second colour at the next step
-    if (active && r >= 1 && r <= TY + 2) LdsRow<T>::store(Bk + boff, gx, ngx, xc);
+#undef MGMC_RB3_UPDATE
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      xmm[q] = xm[q];
-      xm[q] = xc[q];
-      xc[q] = xp[q];
-      fm[q] = fc[q];
-    }
+    for (int q = 0; q < 8; ++q) fm[q] = fc[q];
   }
 }
 

@@ -187,6 +187,7 @@ struct Common {
   // and the number of workgroups it aims for
   int tune_fused = 1;
   int tune_rb_threads = 512;
+  int tune_rb_chains = 1;     // chains per workgroup of k_sweep_rb3 (2: with f shared and K even)
   int tune_rb_blocks = 768;  // 128^3 x 64 chains: three z-segments (longer marches beat fuller last waves)
   int tune_rb_lds_kb = 64;    // LDS budget of its tiles (KB; the device allows up to max_lds)
   int tune_fold_prolong = 1; // fold the prolongation into the first fused post-sweep
@@ -333,15 +334,21 @@ struct Engine : EngineBase {
 
   // ---- fused red-black sweep (k_sweep_rb3) configuration ----
   struct RbShape {
-    int TY, SZ, ntile, nseg, threads;
+    int TY, SZ, ntile, nseg, threads, CB;
     size_t lds;
   };
-  // rows per tile: threads per workgroup and 64 KB of double-buffered LDS
+  // chains per workgroup: 2 when asked for, f is shared by the chains and K is even
+  int rb_cb(int l) const {
+    return sizeof(T) == 4 && C.tune_rb_chains == 2 && geo(l).fcs == 0 && C.K % 2 == 0 ? 2 : 1;
+  }
+  // rows per tile: threads per workgroup and CB x 64 KB of double-buffered LDS
   int rb_ty(int l) const {
     const DevGeom& g = C.L[l]->g;
     int ngx = g.px >> 3;
-    int by_threads = std::min(C.tune_rb_threads, rb3_max_threads<T>()) / ngx - 4;
-    const size_t budget = std::min((size_t)C.tune_rb_lds_kb * 1024, (size_t)C.max_lds);
+    const int cb = rb_cb(l);
+    const int cap = cb > 1 ? rb3_max_threads<T, 2>() : rb3_max_threads<T, 1>();
+    int by_threads = std::min(C.tune_rb_threads, cap) / ngx - 4;
+    const size_t budget = std::min((size_t)C.tune_rb_lds_kb * 1024 * cb, (size_t)C.max_lds) / cb;
     int by_lds = (int)(((budget - 40 * sizeof(T)) / (2 * sizeof(T) * (size_t)g.px) - 6) / 2);
     return std::min(std::min(by_threads, by_lds), g.n1 - 1);
   }
@@ -354,15 +361,17 @@ struct Engine : EngineBase {
     const DevGeom& g = C.L[l]->g;
     RbShape s;
     int ngx = g.px >> 3;
+    s.CB = rb_cb(l);
     s.TY = rb_ty(l);
     s.ntile = (g.n1 - 1 + s.TY - 1) / s.TY;
     int nplanes = g.n2 - 1;
-    int want = (C.tune_rb_blocks + s.ntile * C.K - 1) / (s.ntile * C.K);
+    const int ngroups = C.K / s.CB;
+    int want = (C.tune_rb_blocks + s.ntile * ngroups - 1) / (s.ntile * ngroups);
     int nseg = std::max(1, std::min(want, std::max(1, nplanes / 8)));
     s.SZ = (nplanes + nseg - 1) / nseg;
     s.nseg = (nplanes + s.SZ - 1) / s.SZ;
     s.threads = ((ngx * (s.TY + 4) + 63) / 64) * 64;
-    s.lds = ((size_t)2 * (2 * s.TY + 6) * g.px + 5 * 8) * sizeof(T);  // + the zero guards of k_sweep_rb3
+    s.lds = ((size_t)2 * (2 * s.TY + 6) * g.px + 5 * 8) * sizeof(T) * s.CB;  // + the zero guards of k_sweep_rb3
     return s;
   }
   // prolongate-add pending for the next fused sweep of a level (folded into its loads)
@@ -375,7 +384,7 @@ struct Engine : EngineBase {
     T* in = X(l);
     T* out = cur[l] ? x[l].p : x2[l].p;
     NoiseKey nk = key(tag);
-    dim3 grid((unsigned)(s.ntile * s.nseg * C.K));
+    dim3 grid((unsigned)(s.ntile * s.nseg * (C.K / s.CB)));
     int first = forward ? 0 : 1;
     const bool timed = prof_begin(l);
     const T* xcoarse = nullptr;
@@ -387,15 +396,22 @@ struct Engine : EngineBase {
       alpha = (T)pending_alpha[l];
       pending[l] = 0;
     }
-    if (noise == NOISE_NONE)
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_NONE>, in, out, f[l].p, extp, geo(l), Lv.st, first,
-             s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
-    else if (noise == NOISE_PHILOX)
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_PHILOX>, in, out, f[l].p, extp, geo(l), Lv.st,
-             first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
-    else
-      launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE_EXTERNAL>, in, out, f[l].p, extp, geo(l), Lv.st,
-             first, s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha);
+#define MGMC_RB3(NOISE, CB)                                                                                      \
+  launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE, CB>, in, out, f[l].p, extp, geo(l), Lv.st, first, \
+         s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha)
+    if constexpr (sizeof(T) == 4) {  // chain-blocked FP32 only (rb_cb)
+      if (s.CB == 2) {
+        if (noise == NOISE_NONE) MGMC_RB3(NOISE_NONE, 2);
+        else if (noise == NOISE_PHILOX) MGMC_RB3(NOISE_PHILOX, 2);
+        else MGMC_RB3(NOISE_EXTERNAL, 2);
+      }
+    }
+    if (s.CB == 1) {
+      if (noise == NOISE_NONE) MGMC_RB3(NOISE_NONE, 1);
+      else if (noise == NOISE_PHILOX) MGMC_RB3(NOISE_PHILOX, 1);
+      else MGMC_RB3(NOISE_EXTERNAL, 1);
+    }
+#undef MGMC_RB3
     if (timed) prof_end();
     cur[l] ^= 1;
   }
@@ -462,9 +478,12 @@ struct Engine : EngineBase {
 #ifndef MGMC_EMULATION
     if (C.max_lds <= 65536) return;
     const void* ks[] = {
-        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE>),
-        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX>),
-        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL>),
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE, 1>),
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX, 1>),
+        reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL, 1>),
+        reinterpret_cast<const void*>(k_sweep_rb3<float, NOISE_NONE, 2>),
+        reinterpret_cast<const void*>(k_sweep_rb3<float, NOISE_PHILOX, 2>),
+        reinterpret_cast<const void*>(k_sweep_rb3<float, NOISE_EXTERNAL, 2>),
         reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_NONE>),
         reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_PHILOX>),
         reinterpret_cast<const void*>(k_sweep_box_march<T, NOISE_EXTERNAL>),
@@ -1932,6 +1951,9 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   } else if (k == "rb_blocks") {
     if (value < 1) throw Error(MGMC_ERR_ARG, "rb_blocks must be >= 1");
     h->C.tune_rb_blocks = value;
+  } else if (k == "rb_chains") {
+    if (value != 1 && value != 2) throw Error(MGMC_ERR_ARG, "rb_chains must be 1 or 2");
+    h->C.tune_rb_chains = value;
   } else
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
@@ -1948,6 +1970,7 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
 static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_threads") return h->C.tune_rb_threads;
   if (k == "rb_blocks") return h->C.tune_rb_blocks;
+  if (k == "rb_chains") return h->C.tune_rb_chains;
   if (k == "rr_threads") return h->C.tune_rr_threads;
   if (k == "plane_sweep_stage3") return h->C.tune_planes_stage3;
   throw Error(MGMC_ERR_ARG, "no autotuned key '" + k + "'");
@@ -2001,7 +2024,8 @@ int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms) {
     const char* name;
     std::vector<int> values;
   };
-  std::vector<Key> keys = {{"rb_threads", {256, 384, 512, 768, 1024}},
+  std::vector<Key> keys = {{"rb_chains", {1, 2}},
+                           {"rb_threads", {256, 384, 512, 768, 1024}},
                            {"rb_blocks", {384, 512, 768, 1024, 1536, 2048}},
                            {"rr_threads", {256, 512, 768, 1024}},
                            {"plane_sweep_stage3", {1, 0}}};

@@ -240,6 +240,38 @@ def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
                 assert np.abs(a - c).max() <= tol * np.abs(a).max(), (fw, st, bool(kw))
 
 
+@pytest.mark.parametrize("n,K", [((32, 16, 24), 4), ((24, 20, 16), 6)])
+@pytest.mark.parametrize("threads,blocks", [(512, 1024), (64, 1), (64, 100000)])
+def test_chain_blocked_sweep_matches_one_chain_per_workgroup(n, K, threads, blocks):
+    """Two chains per workgroup of the level-0 red-black sweep (f shared by
+    the chains, FP32) give the sample path of one chain per workgroup: whole
+    sampling cycles (the folded coarse correction and the low-rank terms
+    included) over several tile and z-segment shapes. The arithmetic per
+    chain is the same (on gfx950 the states come out bit-equal); the bound
+    allows for the compiler contracting the two kernels' products apart."""
+    op, _ = make_problem(3, n, 3)
+    p = params(3)
+    N = op.lattice.Nvertex
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal(N)
+    x0 = rng.standard_normal((K, N))
+    states = []
+    for cb in (1, 2):
+        s = mg.MultigridMCSampler(op, p, nchains=K, precision="fp32", seed=21)
+        s.set_tuning("rb_chains", cb)
+        s.set_tuning("rb_threads", threads)
+        s.set_tuning("rb_blocks", blocks)
+        assert s.get_tuning("rb_chains") == cb
+        s.fix_rhs(f)
+        s.set_state(x0)
+        s.run(3, record=False)
+        states.append(s.get_state())
+    a, c = states
+    print(f"chain-blocked sweep, {n} x {K} chains, rb_threads {threads}: max |diff| {np.abs(a - c).max():.3e} "
+          f"(bit-equal: {np.array_equal(a, c)})")
+    assert np.abs(a - c).max() <= 2e-6 * np.abs(a).max()
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
 @pytest.mark.parametrize("stage3", [0, 1])
 def test_plane_sweep_matches_colour_phases(precision, tol, stage3):
