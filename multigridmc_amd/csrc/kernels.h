@@ -4,7 +4,11 @@
 #ifndef MGMC_KERNELS_H
 #define MGMC_KERNELS_H
 
+#ifdef MGMC_EMULATION
+#include "hip_emu.h"  // host emulation of the sanitizer tests (tests/emu)
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include <tuple>

@@ -6,7 +6,11 @@
 // by issuing kernels on one HIP stream; every launch covers all K chains.
 // A whole cycle (plus the observable and counter update) can be captured in
 // a hipGraph and replayed.
+#ifdef MGMC_EMULATION
+#include "hip_emu.h"  // host emulation of the sanitizer tests (tests/emu)
+#else
 #include <hip/hip_runtime.h>
+#endif
 
 #include <algorithm>
 #include <cmath>
