@@ -144,6 +144,55 @@ def cpu_baseline(n, m, seed, budget_s, nthreads):
     }
 
 
+def mixed_batch(args):
+    """The uneven batch of the batched-chains configuration: 16 posterior
+    instances on lattices drawn from {32^3, 48^3, 64^3, 96^3, 128^3} (seeded),
+    8-32 ball measurements each, 4 FP32 chains per instance, V(1,1) SSOR down
+    to at most 8 cells per direction. Each form runs --warmup cycles, then
+    --steps timed cycles (graph replays, device synchronisation on both
+    sides); the line reports chain-samples per second of the whole batch."""
+    import torch
+    import multigridmc_amd as mg
+    from multigridmc_amd import measurements as ms
+    from multigridmc_amd.batch import SamplerBatch
+    rng = np.random.default_rng(2024)
+    inst = []
+    for k in range(16):
+        n = int(rng.choice([32, 48, 64, 96, 128]))
+        lat = mg.Lattice(n, n, n)
+        m = int(rng.integers(8, 33))
+        centres, y, var = mg.seeded_ball_measurements(lat, m, 0.05, 1e-4, 2024 + k)
+        op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
+        inst.append(dict(op=op, params=mg.MultigridParameters(nlevel=ms._nlevel(op)), nchains=4,
+                         f=op.get_B() @ (y / var), observables=[(np.array([op.lattice.Nvertex // 2]), np.array([1.0]))],
+                         seed=77 + k))
+    chains = sum(i["nchains"] for i in inst)
+    out = {"workload": "mixed batch of 16 instances, lattices " +
+                       ",".join(str(i["op"].lattice.n[0]) for i in inst) + ", 4 FP32 chains each",
+           "chains": chains, "steps": args.steps, "warmup": args.warmup}
+    for name, mode, persistent in (("merged", "merged", 0), ("merged_persistent", "merged", 1),
+                                   ("streams", "streams", 0)):
+        b = SamplerBatch(inst, mode=mode, persistent=persistent)
+        bufs = [torch.empty((max(args.steps, args.warmup), s.nchains, 1), dtype=torch.float64, device="cuda")
+                for s in b.samplers]
+        torch.cuda.synchronize()
+        b.run_dev(args.warmup, bufs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run_dev(args.steps, bufs)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        st = b.stats()
+        out[name] = {"ms_per_cycle": 1e3 * t / args.steps, "chain_samples_per_s": chains * args.steps / t}
+        if st:
+            out[name]["launches_per_cycle"] = st[0]
+            out[name]["separate_launches_per_cycle"] = st[1]
+        b.close()
+        del bufs
+    out["speedup_merged_vs_streams"] = out["streams"]["ms_per_cycle"] / out["merged"]["ms_per_cycle"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +214,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--colour-phases", action="store_true",
                     help="sweep level 0 with one launch per colour instead of the single-pass kernel")
+    ap.add_argument("--mixed-batch", action="store_true",
+                    help="only time the mixed batch of 16 instances (32^3..128^3, 8-32 balls, 4 FP32 chains each): "
+                         "merged launches (plain and persistent) against one stream per instance")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -188,6 +240,10 @@ def main():
         ms = e0.elapsed_time(e1) / 20
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copy_ms": ms,
                           "copy_GBps_read_plus_write": 8 * n / (ms * 1e-3) / 1e9}))
+        return
+
+    if args.mixed_batch:
+        mixed_batch(args)
         return
 
     if args.gpus != 1:

@@ -130,6 +130,34 @@ int mgmc_set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const 
 int mgmc_run(mgmc_handle* h, int nsamples, double* obs, int use_graph);
 /* Same, with the observable buffer in device memory (nsamples x nchains x nprobe). */
 int mgmc_run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph);
+
+/* ---- Batches of independent instances (no reference counterpart: the
+ * reference runs one sampler at a time; these are for callers that sample
+ * many problems of different sizes at once) --------------------------------
+ * A batch runs the MGMC cycles of several handles (lattices of any sizes,
+ * any chain counts, FP32 or FP64, each with its own measurements and seed)
+ * together: at every point of the V-cycle the launches all instances make of
+ * one kernel are merged into a single launch whose blocks are the instances'
+ * blocks side by side, and one cycle of the whole batch is one captured
+ * hipGraph. Every instance's samples equal those of its own mgmc_run_dev.
+ * Requirements: handles on one device, multicolour order, constant-
+ * coefficient stencils, V-cycles (MGMC_ERR_UNSUPPORTED otherwise). The
+ * handles must outlive the batch and are used by it between calls. */
+typedef struct mgmc_batch mgmc_batch;
+int mgmc_batch_create(int n, mgmc_handle* const* handles, mgmc_batch** out);
+int mgmc_batch_destroy(mgmc_batch* b);
+/* nsamples cycles of every instance; obs_dev[i] (may be NULL, or the array
+ * NULL) receives instance i's observables as mgmc_run_dev does. Work is
+ * queued on the batch's stream; later work on the handles' streams waits
+ * for it. */
+int mgmc_batch_run_dev(mgmc_batch* b, int nsamples, double* const* obs_dev, int use_graph);
+/* Persistent merged launches: at most workgroups_per_cu workgroups per CU
+ * per launch, each looping over the blocks of all instances (0, the
+ * default: one workgroup per block). */
+int mgmc_batch_set_persistent(mgmc_batch* b, int workgroups_per_cu);
+/* Launches per batch cycle after merging, and what the instances issue on
+ * their own (either pointer may be NULL). */
+int mgmc_batch_stats(const mgmc_batch* b, int64_t* launches_per_cycle, int64_t* separate_launches_per_cycle);
 /* Pointwise posterior statistics (posterior_statistics,
  * src/measurements/posterior_statistics.cc:4-55): runs `nsamples` cycles from
  * the current state and returns the mean and variance of every fine vertex

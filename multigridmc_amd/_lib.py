@@ -73,6 +73,11 @@ _SIGS = {
     "mgmc_run": [_vp, C.c_int, _vp, C.c_int],
     "mgmc_run_dev": [_vp, C.c_int, _vp, C.c_int],
     "mgmc_moments": [_vp, C.c_int, _vp, _vp],
+    "mgmc_batch_create": [C.c_int, _vp, C.POINTER(_vp)],
+    "mgmc_batch_destroy": [_vp],
+    "mgmc_batch_run_dev": [_vp, C.c_int, _vp, C.c_int],
+    "mgmc_batch_set_persistent": [_vp, C.c_int],
+    "mgmc_batch_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "mgmc_set_counter": [_vp, C.c_uint64],
     "mg_precondition": [_vp, _vp, _vp],
     "mg_solve": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, _vp, C.POINTER(C.c_int)],

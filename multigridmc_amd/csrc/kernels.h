@@ -11,9 +11,11 @@
 #endif
 #include <stdint.h>
 
+#include <cstring>
 #include <tuple>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "philox.h"
 
@@ -62,6 +64,85 @@ struct NoiseKey {
 
 enum NoiseMode { NOISE_NONE = 0, NOISE_PHILOX = 1, NOISE_EXTERNAL = 2 };
 
+// The workgroup a kernel body works on: its block coordinates and the grid
+// of its own launch. A plain launch passes the hardware's (Blk::hw()); a
+// merged launch of several instances (k_multi) passes each instance's own.
+struct Blk {
+  uint32_t x, y, gx, gy;
+  __device__ static Blk hw() { return Blk{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}; }
+};
+
+// ---------------------------------------------------------------------------
+// Merged launches. A kernel whose body is a __device__ function of a Blk and
+// its parameters is "batchable": the launches that several independent
+// instances (lattices of different sizes, each with its own fields and
+// hierarchy) make of it at the same point of the cycle become one launch of
+// k_multi, which reads the instance's arguments from a table. Item i owns the
+// merged blocks [start_i, start_i + gx_i gy_i); starts are multiples of 8, so
+// that a block keeps the XCD (block index mod 8) it has in a plain launch.
+// With fewer workgroups than blocks (`total`), the workgroups loop over the
+// blocks (persistent form); bodies then must reach every barrier on all
+// threads, which holds for every batchable body of this library.
+// ---------------------------------------------------------------------------
+template <typename Tup>
+struct MultiItem {
+  uint32_t start, gx, gy, pad;
+  Tup args;
+};
+template <auto Body, typename Tup, size_t... I>
+__device__ __forceinline__ void multi_call(const Blk& bk, const Tup& t, std::index_sequence<I...>) {
+  Body(bk, std::get<I>(t)...);
+}
+template <auto Body, int MAXT, typename Tup>
+__global__ void __launch_bounds__(MAXT) k_multi(const MultiItem<Tup>* __restrict__ items, int n, uint32_t total) {
+  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    if (b != blockIdx.x) __syncthreads();  // the previous block's LDS is consumed
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].start <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const MultiItem<Tup>* it = items + lo;
+    const uint32_t local = b - it->start;
+    if (local >= it->gx * it->gy) continue;  // alignment padding between items
+    const Blk bk{local % it->gx, local / it->gx, it->gx, it->gy};
+    multi_call<Body>(bk, it->args, std::make_index_sequence<std::tuple_size<Tup>::value>{});
+  }
+}
+
+// A launch recorded for merging: the k_multi kernel of its body, its shape
+// and one MultiItem (start 0) holding its arguments.
+using MultiLauncher = void (*)(dim3 grid, dim3 block, size_t shmem, hipStream_t stream, const void* items, int n,
+                               uint32_t total);
+struct LaunchRec {
+  const void* multi = nullptr;  // k_multi instantiation (identifies the body)
+  MultiLauncher launcher = nullptr;
+  bool tolerant = false;  // the body computes the same with more threads per block than asked for
+  dim3 grid, block;
+  size_t shmem = 0;
+  std::vector<char> item;
+  size_t item_align = 8;
+  int dir = 0, level = 0, stage = 0, seq = 0, instance = 0;
+};
+// Launches made while a recorder is installed (on this thread) are recorded
+// instead of issued; dir / level / stage say where in the cycle they are
+// (set by the engine), seq numbers launches within one (dir, level, stage).
+struct Recorder {
+  std::vector<LaunchRec> recs;
+  int dir = 0, level = 0, stage = 0;
+  int instance = 0;
+};
+extern thread_local Recorder* g_rec;
+inline void rec_mark(int dir, int level, int stage) {
+  if (g_rec) {
+    g_rec->dir = dir;
+    g_rec->level = level;
+    g_rec->stage = stage;
+  }
+}
+[[noreturn]] void unbatchable_launch();
+
 // Kernel launch through hipLaunchKernel with each argument converted to the
 // kernel's parameter type (one entry point for every launch of the engine).
 template <typename Tuple, size_t... I>
@@ -71,6 +152,7 @@ inline void launch_args(Tuple& t, void** argv, std::index_sequence<I...>) {
 template <typename... P, typename... A>
 inline void launch(dim3 grid, dim3 block, size_t shmem, hipStream_t stream, void (*k)(P...), A&&... args) {
   static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  if (g_rec) unbatchable_launch();
 #ifdef MGMC_EMULATION
   (void)stream;
   emu_launch(grid, block, shmem, k, std::forward<A>(args)...);
@@ -81,6 +163,55 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, hipStream_t stream, void
   hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, argv, shmem, stream);
   if (e != hipSuccess) launch_failed(e);
 #endif
+}
+
+// A batchable kernel: its body, the workgroup size bound of its kernels,
+// whether the body tolerates larger workgroups, and its plain kernel (which
+// calls the body with Blk::hw()). Engines launch through these tags.
+template <auto Body, int MAXT, bool TOL, auto Kern>
+struct Batchable {};
+
+template <auto Body, int MAXT, typename Tup>
+void multi_launch(dim3 grid, dim3 block, size_t shmem, hipStream_t stream, const void* items, int n, uint32_t total) {
+  auto saved = g_rec;
+  g_rec = nullptr;
+  launch(grid, block, shmem, stream, k_multi<Body, MAXT, Tup>, static_cast<const MultiItem<Tup>*>(items), n, total);
+  g_rec = saved;
+}
+template <auto Body, int MAXT, bool TOL, typename... P, typename... A>
+void record_launch(void (*)(P...), dim3 grid, dim3 block, size_t shmem, A&&... args) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  using Tup = std::tuple<std::decay_t<P>...>;
+  MultiItem<Tup> it{0u, grid.x, grid.y, 0u, Tup(static_cast<P>(args)...)};
+  LaunchRec r;
+  r.multi = reinterpret_cast<const void*>(k_multi<Body, MAXT, Tup>);
+  r.launcher = &multi_launch<Body, MAXT, Tup>;
+  r.tolerant = TOL;
+  r.grid = grid;
+  r.block = block;
+  r.shmem = shmem;
+  r.item.resize(sizeof(it));
+  std::memcpy(r.item.data(), static_cast<const void*>(&it), sizeof(it));
+  r.item_align = alignof(MultiItem<Tup>);
+  Recorder& R = *g_rec;
+  r.dir = R.dir;
+  r.level = R.level;
+  r.stage = R.stage;
+  r.instance = R.instance;
+  r.seq = 0;
+  for (const LaunchRec& q : R.recs)
+    if (q.dir == r.dir && q.level == r.level && q.stage == r.stage) ++r.seq;
+  R.recs.push_back(std::move(r));
+}
+template <auto Body, int MAXT, bool TOL, auto Kern, typename... A>
+inline void launch(dim3 grid, dim3 block, size_t shmem, hipStream_t stream, Batchable<Body, MAXT, TOL, Kern>,
+                   A&&... args) {
+  if (g_rec) {
+    if (grid.z != 1) unbatchable_launch();
+    record_launch<Body, MAXT, TOL>(Kern, grid, block, shmem, std::forward<A>(args)...);
+    return;
+  }
+  launch(grid, block, shmem, stream, Kern, std::forward<A>(args)...);
 }
 
 // Four N(0,1) normals of one Philox call (Box-Muller on two uniform pairs).

@@ -285,11 +285,11 @@ struct ProlongStream {
 // STAR: colour = (i+j+k)&1; box: colour = (i&1) | (j&1)<<1 | (k&1)<<2.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM, bool STAR, int NOISE>
-__global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __restrict__ f,
+__device__ __forceinline__ void b_sweep(const Blk bk, T* __restrict__ x, const T* __restrict__ f,
                                                const T* __restrict__ ext, DevGeom g, Stencil st, int colour,
                                                int nrows, int K, NoiseKey nk) {
   const int ngx = g.px >> 3;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = (long long)bk.x * blockDim.x + threadIdx.x;
   int gx = (int)(t % ngx);
   long long row = t / ngx;
   // groups holding no interior vertex (the x padding) change nothing
@@ -360,6 +360,14 @@ __global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __res
     box_update4<1, T, DIM, NOISE>(xc, xr, g, st, fb, xi, eg, 8 * gx);
   store8<T>(xr, xc + 1);
 }
+template <typename T, int DIM, bool STAR, int NOISE>
+__global__ void __launch_bounds__(256) k_sweep(T* __restrict__ x, const T* __restrict__ f,
+                                               const T* __restrict__ ext, DevGeom g, Stencil st, int colour,
+                                               int nrows, int K, NoiseKey nk) {
+  b_sweep<T, DIM, STAR, NOISE>(Blk::hw(), x, f, ext, g, st, colour, nrows, K, nk);
+}
+template <typename T, int DIM, bool STAR, int NOISE>
+constexpr Batchable<b_sweep<T, DIM, STAR, NOISE>, 256, false, k_sweep<T, DIM, STAR, NOISE>> kb_sweep{};
 
 // ---------------------------------------------------------------------------
 // Variable coefficients: every vertex has its own stencil, stored as planes
@@ -595,13 +603,13 @@ constexpr int MMAX = 128;
 // read once and the chains' x loads are independent.
 constexpr int LR_CPB = 1;
 template <typename T>
-__global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
+__device__ __forceinline__ void b_lr_dot(const Blk bk, const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
                                                 const long long* __restrict__ bidx, const double* __restrict__ bval,
                                                 int m, int K, double* __restrict__ z, int nsplit) {
   // block (kcol * nsplit + part, chain group): part `part` of nsplit equal
   // runs of the column's entries; the partial sums are added in order by
   // the solve
-  const int kcol = blockIdx.x / nsplit, part = blockIdx.x % nsplit, c0 = blockIdx.y * LR_CPB;
+  const int kcol = bk.x / nsplit, part = bk.x % nsplit, c0 = bk.y * LR_CPB;
   const int nc = K - c0 < LR_CPB ? K - c0 : LR_CPB;
   double s[LR_CPB];
 #pragma unroll
@@ -651,15 +659,23 @@ __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long lo
     z[((long long)(c0 + q) * m + kcol) * nsplit + part] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
+                                                const long long* __restrict__ bidx, const double* __restrict__ bval,
+                                                int m, int K, double* __restrict__ z, int nsplit) {
+  b_lr_dot<T>(Blk::hw(), x, cs, bptr, bidx, bval, m, K, z, nsplit);
+}
+template <typename T>
+constexpr Batchable<b_lr_dot<T>, 256, false, k_lr_dot<T>> kb_lr_dot{};
 
 // one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c)
 template <int NOISE>
-__global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z, const double* __restrict__ M,
+__device__ __forceinline__ void b_lr_solve(const Blk bk, const double* __restrict__ z, const double* __restrict__ M,
                                                    const double* __restrict__ sqrt_sigma, double zc,
                                                    const double* __restrict__ ext_lr, double* __restrict__ u, int m,
                                                    NoiseKey nk, int nsplit) {
   __shared__ double a[MMAX];
-  const int c = blockIdx.x, tid = threadIdx.x;
+  const int c = bk.x, tid = threadIdx.x;
   if (tid < m) {
     double zs = 0.0;
     for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + tid) * nsplit + p];
@@ -679,20 +695,29 @@ __global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z,
     u[(long long)c * m + tid] = acc;
   }
 }
+template <int NOISE>
+__global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z, const double* __restrict__ M,
+                                                   const double* __restrict__ sqrt_sigma, double zc,
+                                                   const double* __restrict__ ext_lr, double* __restrict__ u, int m,
+                                                   NoiseKey nk, int nsplit) {
+  b_lr_solve<NOISE>(Blk::hw(), z, M, sqrt_sigma, zc, ext_lr, u, m, nk, nsplit);
+}
+template <int NOISE>
+constexpr Batchable<b_lr_solve<NOISE>, MMAX, false, k_lr_solve<NOISE>> kb_lr_solve{};
 
 // x_c[rows[r]] += (R u_c)_r for the LR_CPU chains of a block (grid: row
 // blocks x chain groups); a row's entries are read once for all its chains.
 constexpr int LR_CPU = 8;
 template <typename T>
-__global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
+__device__ __forceinline__ void b_lr_update(const Blk bk, T* __restrict__ x, long long cs, const long long* __restrict__ rows,
                                                    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                    const double* __restrict__ val, int nrows,
                                                    const double* __restrict__ u, int m, int K) {
   __shared__ double us[LR_CPU][MMAX];
-  const int c0 = blockIdx.y * LR_CPU;
+  const int c0 = bk.y * LR_CPU;
   const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
   const int tid = threadIdx.x;
-  const int r = blockIdx.x * blockDim.x + tid;
+  const int r = bk.x * blockDim.x + tid;
   const bool ok = r < nrows;
   // the row's x values and W entries are requested before the block waits
   // for u in LDS (latency-bound scatter)
@@ -723,13 +748,22 @@ __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long 
   for (int q = 0; q < LR_CPU; ++q)
     if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
+                                                   const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                   const double* __restrict__ val, int nrows,
+                                                   const double* __restrict__ u, int m, int K) {
+  b_lr_update<T>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, u, m, K);
+}
+template <typename T>
+constexpr Batchable<b_lr_update<T>, 256, false, k_lr_update<T>> kb_lr_update{};
 
 // k_lr_solve and k_lr_update in one launch: every block first forms u for
 // its LR_CPU chains (u_c = M (zc z_c + sqrt(Sigma) xi'_c), the same sums in
 // the same order as k_lr_solve, recomputed per row block: m^2 FMAs per chain),
 // then applies x_c[rows[r]] += (R u_c)_r.
 template <typename T, int NOISE>
-__global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long long cs,
+__device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ x, long long cs,
                                                          const long long* __restrict__ rows,
                                                          const int32_t* __restrict__ ptr,
                                                          const int32_t* __restrict__ col,
@@ -740,10 +774,10 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
                                                          int nsplit) {
   __shared__ double a[LR_CPU][MMAX];
   __shared__ double us[LR_CPU][MMAX];
-  const int c0 = blockIdx.y * LR_CPU;
+  const int c0 = bk.y * LR_CPU;
   const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
   const int tid = threadIdx.x;
-  const int r = blockIdx.x * blockDim.x + tid;
+  const int r = bk.x * blockDim.x + tid;
   const bool ok = r < nrows;
   long long rr = 0;
   int e0 = 0, e1 = 0;
@@ -792,6 +826,20 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
   for (int q = 0; q < LR_CPU; ++q)
     if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
 }
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long long cs,
+                                                         const long long* __restrict__ rows,
+                                                         const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ col,
+                                                         const double* __restrict__ val, int nrows,
+                                                         const double* __restrict__ z, const double* __restrict__ M,
+                                                         const double* __restrict__ sqrt_sigma, double zc,
+                                                         const double* __restrict__ ext_lr, int m, int K, NoiseKey nk,
+                                                         int nsplit) {
+  b_lr_solve_update<T, NOISE>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, z, M, sqrt_sigma, zc, ext_lr, m, K, nk, nsplit);
+}
+template <typename T, int NOISE>
+constexpr Batchable<b_lr_solve_update<T, NOISE>, 256, false, k_lr_solve_update<T, NOISE>> kb_lr_solve_update{};
 
 // ---------------------------------------------------------------------------
 // Transfers. Grids are (groups x rows, chains): thread t of chain
@@ -805,9 +853,9 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
 // divides by 8 the (block, chain) items are dealt out such that an XCD runs
 // consecutive row blocks of one chain, whose shared neighbour rows and
 // planes then meet in its L2.
-__device__ __forceinline__ void xcd_rowblock(int& bx, int& c) {
-  const int gxn = (int)gridDim.x, nb = (int)(gridDim.x * gridDim.y);
-  int lin = (int)blockIdx.x + (int)blockIdx.y * gxn;
+__device__ __forceinline__ void xcd_rowblock(const Blk& bk, int& bx, int& c) {
+  const int gxn = (int)bk.gx, nb = (int)(bk.gx * bk.gy);
+  int lin = (int)bk.x + (int)bk.y * gxn;
   if ((nb & 7) == 0) lin = (lin & 7) * (nb >> 3) + (lin >> 3);
   bx = lin % gxn;
   c = lin / gxn;
@@ -816,12 +864,12 @@ __device__ __forceinline__ void xcd_rowblock(int& bx, int& c) {
 // r = f - A x on interior vertices (the sparse part of
 // src/linear_operator/linear_operator.hh:78-88, multigridmc_sampler.cc:125-126)
 template <typename T, int DIM, bool STAR>
-__global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const T* __restrict__ f, T* __restrict__ r,
+__device__ __forceinline__ void b_residual(const Blk bk, const T* __restrict__ x, const T* __restrict__ f, T* __restrict__ r,
                                                   DevGeom g, Stencil st) {
   const int ngx = g.px >> 3;
   const int nrows = (g.n1 - 1) * (DIM == 3 ? g.n2 - 1 : 1);
   int bx, c;
-  xcd_rowblock(bx, c);
+  xcd_rowblock(bk, bx, c);
   const int t = bx * blockDim.x + threadIdx.x;
   if (t >= ngx * nrows) return;
   const int gx = t % ngx, row = t / ngx;
@@ -878,17 +926,24 @@ __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const
   }
   store8<T>(r + base, out);
 }
+template <typename T, int DIM, bool STAR>
+__global__ void __launch_bounds__(256) k_residual(const T* __restrict__ x, const T* __restrict__ f, T* __restrict__ r,
+                                                  DevGeom g, Stencil st) {
+  b_residual<T, DIM, STAR>(Blk::hw(), x, f, r, g, st);
+}
+template <typename T, int DIM, bool STAR>
+constexpr Batchable<b_residual<T, DIM, STAR>, 256, false, k_residual<T, DIM, STAR>> kb_residual{};
 
 // fc[I] = sum_sigma p(sigma) r[2I + sigma], p = prod {1/2, 1, 1/2}
 // (IntergridOperator::restrict, src/intergrid/intergrid_operator.hh:74-88).
 // A thread owns four coarse vertices I = 4 gq .. 4 gq + 3 of a coarse row,
 // i.e. fine x positions 8 gq - 1 .. 8 gq + 7.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc) {
+__device__ __forceinline__ void b_restrict(const Blk bk, const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc) {
   const int nq = (gc.n0 - 1 + 4) / 4;  // groups of four coarse vertices covering I = 0 .. n0c - 1
   const int nrows = (gc.n1 - 1) * (DIM == 3 ? gc.n2 - 1 : 1);
   int bx, c;
-  xcd_rowblock(bx, c);
+  xcd_rowblock(bk, bx, c);
   const int t = bx * blockDim.x + threadIdx.x;
   if (t >= nq * nrows) return;
   const int gq = t % nq, row = t / nq;
@@ -918,17 +973,23 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __
     if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a];
   }
 }
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, DevGeom gf, DevGeom gc) {
+  b_restrict<T, DIM>(Blk::hw(), r, fc, gf, gc);
+}
+template <typename T, int DIM>
+constexpr Batchable<b_restrict<T, DIM>, 256, false, k_restrict<T, DIM>> kb_restrict{};
 
 // x += alpha P xc (IntergridOperator::prolongate_add in gather form,
 // src/intergrid/intergrid_operator.hh:106-120). A thread owns an aligned
 // 8-vertex fine group, whose coarse neighbours are I = 4 gx .. 4 gx + 4.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* __restrict__ xc, DevGeom gf, DevGeom gc,
+__device__ __forceinline__ void b_prolongate(const Blk bk, T* __restrict__ x, const T* __restrict__ xc, DevGeom gf, DevGeom gc,
                                                     T alpha) {
   const int ngx = gf.px >> 3;
   const int nrows = (gf.n1 - 1) * (DIM == 3 ? gf.n2 - 1 : 1);
   int bx, c;
-  xcd_rowblock(bx, c);
+  xcd_rowblock(bk, bx, c);
   const int t = bx * blockDim.x + threadIdx.x;
   if (t >= ngx * nrows) return;
   const int gx = t % ngx, row = t / ngx;
@@ -961,12 +1022,25 @@ __global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* 
   }
   store8<T>(xr, v);
 }
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256) k_prolongate(T* __restrict__ x, const T* __restrict__ xc, DevGeom gf, DevGeom gc,
+                                                    T alpha) {
+  b_prolongate<T, DIM>(Blk::hw(), x, xc, gf, gc, alpha);
+}
+template <typename T, int DIM>
+constexpr Batchable<b_prolongate<T, DIM>, 256, false, k_prolongate<T, DIM>> kb_prolongate{};
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_copy(T* __restrict__ dst, const T* __restrict__ src, long long n) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_copy(const Blk bk, T* __restrict__ dst, const T* __restrict__ src, long long n) {
+  long long t = (long long)bk.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = src[t];
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy(T* __restrict__ dst, const T* __restrict__ src, long long n) {
+  b_copy<T>(Blk::hw(), dst, src, n);
+}
+template <typename T>
+constexpr Batchable<b_copy<T>, 256, false, k_copy<T>> kb_copy{};
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_add(T* __restrict__ dst, const T* __restrict__ src, long long n) {
@@ -975,10 +1049,16 @@ __global__ void __launch_bounds__(256) k_add(T* __restrict__ dst, const T* __res
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill(T* __restrict__ p, long long n, T v) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_fill(const Blk bk, T* __restrict__ p, long long n, T v) {
+  long long t = (long long)bk.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T* __restrict__ p, long long n, T v) {
+  b_fill<T>(Blk::hw(), p, n, v);
+}
+template <typename T>
+constexpr Batchable<b_fill<T>, 256, false, k_fill<T>> kb_fill{};
 
 // reference order (interior, x fastest, FP64) -> padded level layout
 template <typename T>
@@ -1022,14 +1102,14 @@ __global__ void __launch_bounds__(256) k_unpack(const T* __restrict__ pad, doubl
 // (element (r, j) at j * Nc + r) so that a wave reads 64 consecutive rows of
 // one column. LDS: max(2 CPT Nc, P CPT 64) doubles.
 template <typename T, int NOISE, int CPT>
-__global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
+__device__ __forceinline__ void b_coarse(const Blk bk, T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
                                                  const long long* __restrict__ pidx, const double* __restrict__ LT,
                                                  const double* __restrict__ Q, int Nc, int K,
                                                  const double* __restrict__ ext, NoiseKey nk) {
   MGMC_DYNAMIC_SHARED(double, sh);
   double* fs = sh;             // [CPT][Nc]
   double* xs = sh + CPT * Nc;  // [CPT][Nc]
-  const int c0 = blockIdx.y * CPT;
+  const int c0 = bk.y * CPT;
   const int tid = threadIdx.x;
   for (int e = tid; e < CPT * Nc; e += blockDim.x) {
     const int q = e / Nc, j = e % Nc, c = c0 + q;
@@ -1050,7 +1130,7 @@ __global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __r
   __syncthreads();
   const int P = (int)blockDim.x >> 6;
   const int lane = tid & 63, part = tid >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  const int r = bk.x * 64 + lane;
   const int j0 = (Nc * part) / P, j1 = (Nc * (part + 1)) / P;
   double s[CPT];
 #pragma unroll
@@ -1083,6 +1163,15 @@ __global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __r
     }
   }
 }
+template <typename T, int NOISE, int CPT>
+__global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __restrict__ f, long long cs, long long fcs,
+                                                 const long long* __restrict__ pidx, const double* __restrict__ LT,
+                                                 const double* __restrict__ Q, int Nc, int K,
+                                                 const double* __restrict__ ext, NoiseKey nk) {
+  b_coarse<T, NOISE, CPT>(Blk::hw(), x, f, cs, fcs, pidx, LT, Q, Nc, K, ext, nk);
+}
+template <typename T, int NOISE, int CPT>
+constexpr Batchable<b_coarse<T, NOISE, CPT>, 1024, false, k_coarse<T, NOISE, CPT>> kb_coarse{};
 
 // The same coarsest-level products on the FP64 matrix cores. A workgroup
 // owns one 16 x 16 output tile (16 rows r0 .. r0 + 15 x 16 chains c0 ..),
@@ -1097,7 +1186,7 @@ __global__ void __launch_bounds__(1024) k_coarse(T* __restrict__ x, const T* __r
 #ifndef MGMC_EMULATION
 typedef double mgmc_d4 __attribute__((ext_vector_type(4)));
 template <typename T, int NOISE>
-__global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T* __restrict__ f, long long cs,
+__device__ __forceinline__ void b_coarse_mfma(const Blk bk, T* __restrict__ x, const T* __restrict__ f, long long cs,
                                                      long long fcs, const long long* __restrict__ pidx,
                                                      const double* __restrict__ LT, const double* __restrict__ Q,
                                                      int Nc, int K, const double* __restrict__ ext, NoiseKey nk) {
@@ -1106,7 +1195,7 @@ __global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T*
   double* fs = sh;                  // [16][Nc4]
   double* xs = sh + 16 * Nc4;       // [16][Nc4]
   double* red = sh + 32 * Nc4;      // [waves][256]
-  const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+  const int r0 = bk.x * 16, c0 = bk.y * 16;
   const int tid = threadIdx.x, nw = (int)blockDim.x >> 6, w = tid >> 6, lane = tid & 63;
   for (int e = tid; e < 16 * Nc4; e += blockDim.x) {
     const int q = e / Nc4, j = e % Nc4, c = c0 + q;
@@ -1166,6 +1255,15 @@ __global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T*
     }
   }
 }
+template <typename T, int NOISE>
+__global__ void __launch_bounds__(512) k_coarse_mfma(T* __restrict__ x, const T* __restrict__ f, long long cs,
+                                                     long long fcs, const long long* __restrict__ pidx,
+                                                     const double* __restrict__ LT, const double* __restrict__ Q,
+                                                     int Nc, int K, const double* __restrict__ ext, NoiseKey nk) {
+  b_coarse_mfma<T, NOISE>(Blk::hw(), x, f, cs, fcs, pidx, LT, Q, Nc, K, ext, nk);
+}
+template <typename T, int NOISE>
+constexpr Batchable<b_coarse_mfma<T, NOISE>, 512, false, k_coarse_mfma<T, NOISE>> kb_coarse_mfma{};
 #endif
 
 // ---------------------------------------------------------------------------
@@ -1203,7 +1301,7 @@ constexpr int rb3_max_threads() {
 // per-chain code is a macro over separately named arrays, which the compiler
 // keeps in VGPRs (arrays indexed by the chain went to scratch memory).
 template <typename T, int NOISE, int CB>
-__global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
+__device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
                                                     NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
@@ -1226,7 +1324,7 @@ __global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const 
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int r = tid / ngx, gx = tid % ngx;
-  int b = blockIdx.x;
+  int b = bk.x;
   const int tile = b % ntile;
   b /= ntile;
   const int seg = b % nseg;
@@ -1372,6 +1470,16 @@ __global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const 
     for (int q = 0; q < 8; ++q) fm[q] = fc[q];
   }
 }
+template <typename T, int NOISE, int CB>
+__global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
+                                                    const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
+                                                    Stencil st, int first, int TY, int SZ, int ntile, int nseg,
+                                                    NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
+                                                    T alpha) {
+  b_sweep_rb3<T, NOISE, CB>(Blk::hw(), x, xout, f, ext, g, st, first, TY, SZ, ntile, nseg, nk, xcoarse, gc, alpha);
+}
+template <typename T, int NOISE, int CB>
+constexpr Batchable<b_sweep_rb3<T, NOISE, CB>, (rb3_max_threads<T, CB>()), true, k_sweep_rb3<T, NOISE, CB>> kb_sweep_rb3{};
 
 // ---------------------------------------------------------------------------
 // Half of an 8-colour SOR/Gibbs sweep of a 3D box-stencil (27-point) level:
@@ -1385,7 +1493,7 @@ __global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const 
 // the waves of the row parity a colour phase does not touch skip it whole.
 // ---------------------------------------------------------------------------
 template <typename T, int NOISE, bool STAGE3>
-__global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
+__device__ __forceinline__ void b_sweep_box_planes(const Blk bk, T* __restrict__ x, const T* __restrict__ f,
                                                            const T* __restrict__ ext, DevGeom g, StencilT<T> st,
                                                            const T* __restrict__ coef, int kpar, int forward,
                                                            int nplanes, int hth, int xzero, NoiseKey nk) {
@@ -1406,8 +1514,8 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
   // (plane, chain) items so that an XCD runs consecutive planes of a chain,
   // whose shared neighbour planes then meet in its L2 (FP32 cycle at 128^3
   // x 64 chains: -1.2 % against the plain order)
-  int w = (int)blockIdx.x;
-  const int nb = (int)gridDim.x;
+  int w = (int)bk.x;
+  const int nb = (int)bk.gx;
   if ((nb & 7) == 0) w = (w & 7) * (nb >> 3) + (w >> 3);
   const int pidx = w % nplanes;
   const int chain = w / nplanes;
@@ -1500,6 +1608,15 @@ __global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, co
     store8<T>(x + base, v);
   }
 }
+template <typename T, int NOISE, bool STAGE3>
+__global__ void __launch_bounds__(1024) k_sweep_box_planes(T* __restrict__ x, const T* __restrict__ f,
+                                                           const T* __restrict__ ext, DevGeom g, StencilT<T> st,
+                                                           const T* __restrict__ coef, int kpar, int forward,
+                                                           int nplanes, int hth, int xzero, NoiseKey nk) {
+  b_sweep_box_planes<T, NOISE, STAGE3>(Blk::hw(), x, f, ext, g, st, coef, kpar, forward, nplanes, hth, xzero, nk);
+}
+template <typename T, int NOISE, bool STAGE3>
+constexpr Batchable<b_sweep_box_planes<T, NOISE, STAGE3>, 1024, true, k_sweep_box_planes<T, NOISE, STAGE3>> kb_sweep_box_planes{};
 
 // ---------------------------------------------------------------------------
 // One whole 8-colour SOR/Gibbs direction of a 3D box-stencil (27-point) level
@@ -1661,7 +1778,7 @@ __global__ void __launch_bounds__(1024) k_sweep_box_march(const T* __restrict__ 
 // done. One thread per fine 8-vertex group and fine row.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
+__device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T* __restrict__ x, const T* __restrict__ f,
                                                                   T* __restrict__ fc, DevGeom g, DevGeom gc,
                                                                   Stencil st, int TYc, int SZc, int ntile, int nseg) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
@@ -1672,7 +1789,7 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int rt = tid / ngx, gx = tid % ngx;
-  int b = blockIdx.x;
+  int b = bk.x;
   const int tile = b % ntile;
   b /= ntile;
   const int seg = b % nseg;
@@ -1772,24 +1889,42 @@ __global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
+                                                                  T* __restrict__ fc, DevGeom g, DevGeom gc,
+                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg) {
+  b_residual_restrict_star3<T>(Blk::hw(), x, f, fc, g, gc, st, TYc, SZc, ntile, nseg);
+}
+template <typename T>
+constexpr Batchable<b_residual_restrict_star3<T>, rb3_max_threads<T>(), true, k_residual_restrict_star3<T>> kb_residual_restrict_star3{};
 
 // obs[row][c][p] = x_c[probe_p], row taken from the device run counter
 // obs[row][c][p] = sum_e w_e x_c[idx_e] over the entries of observable p
 // (a probe vertex is one entry of weight 1; a ball average is a measurement
 // vector), row taken from the device run counter. One wave per (p, c).
 template <typename T>
-__global__ void __launch_bounds__(64) k_observe(const T* __restrict__ x, long long cs,
+__device__ __forceinline__ void b_observe(const Blk bk, const T* __restrict__ x, long long cs,
                                                 const int32_t* __restrict__ optr,
                                                 const long long* __restrict__ oidx, const double* __restrict__ ow,
                                                 int P, int K, double* __restrict__ obs,
                                                 const uint32_t* __restrict__ ctr) {
-  const int p = blockIdx.x, c = blockIdx.y;
+  const int p = bk.x, c = bk.y;
   const T* xc = x + (long long)c * cs;
   double s = 0.0;
   for (int e = optr[p] + (int)threadIdx.x; e < optr[p + 1]; e += 64) s += ow[e] * (double)xc[oidx[e]];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (threadIdx.x == 0) obs[((long long)ctr[1] * K + c) * P + p] = s;
 }
+template <typename T>
+__global__ void __launch_bounds__(64) k_observe(const T* __restrict__ x, long long cs,
+                                                const int32_t* __restrict__ optr,
+                                                const long long* __restrict__ oidx, const double* __restrict__ ow,
+                                                int P, int K, double* __restrict__ obs,
+                                                const uint32_t* __restrict__ ctr) {
+  b_observe<T>(Blk::hw(), x, cs, optr, oidx, ow, P, K, obs, ctr);
+}
+template <typename T>
+constexpr Batchable<b_observe<T>, 64, false, k_observe<T>> kb_observe{};
 
 // m1[i] += sum_c x_c[i], m2[i] += sum_c x_c[i]^2 over the padded level-0 box
 // (pointwise moments for the posterior mean and variance fields)
@@ -1809,12 +1944,16 @@ __global__ void __launch_bounds__(256) k_moments(const T* __restrict__ x, long l
 }
 
 // advance the sample counter and the observable row after a cycle
-__global__ void k_advance(uint32_t* ctr) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+__device__ __forceinline__ void b_advance(const Blk bk, uint32_t* ctr) {
+  if (threadIdx.x == 0 && bk.x == 0) {
     ctr[0] += 1u;
     ctr[1] += 1u;
   }
 }
+__global__ void __launch_bounds__(1024) k_advance(uint32_t* ctr) {
+  b_advance(Blk::hw(), ctr);
+}
+constexpr Batchable<b_advance, 1024, false, k_advance> kb_advance{};
 
 // The normal the sweeps draw at every fine vertex (reference order)
 __global__ void k_normals(double* __restrict__ out, DevGeom g, int K, NoiseKey nk) {

@@ -40,6 +40,13 @@ static unsigned nblocks(long long n, int bs = 256) { return (unsigned)((n + bs -
 
 void launch_failed(hipError_t e) { throw Error(MGMC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e)); }
 
+thread_local Recorder* g_rec = nullptr;
+void unbatchable_launch() {
+  throw Error(MGMC_ERR_UNSUPPORTED,
+              "a kernel of this configuration cannot run merged with other instances (batches take constant "
+              "coefficients and the multicolour order)");
+}
+
 // Device memory of mgmc_contract_noalloc: the per-block partial sums of the
 // contraction live in this fixed array instead of a buffer allocated per
 // call. Each GPU the library runs on holds its own copy (512 KiB) until the
@@ -401,7 +408,7 @@ struct Engine : EngineBase {
       pending[l] = 0;
     }
 #define MGMC_RB3(NOISE, CB)                                                                                      \
-  launch(grid, s.threads, s.lds, C.stream, k_sweep_rb3<T, NOISE, CB>, in, out, f[l].p, extp, geo(l), Lv.st, first, \
+  launch(grid, s.threads, s.lds, C.stream, kb_sweep_rb3<T, NOISE, CB>, in, out, f[l].p, extp, geo(l), Lv.st, first, \
          s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha)
     if constexpr (sizeof(T) == 4) {  // chain-blocked FP32 only (rb_cb)
       if (s.CB == 2) {
@@ -540,7 +547,7 @@ struct Engine : EngineBase {
       dim3 grid((unsigned)(nplanes * C.K));
       const int xzf = xz ? (h == 0 ? 1 : 2) : 0;
 #define MGMC_PLANES(NOISE, S3)                                                                                  \
-  launch(grid, threads, lds, C.stream, k_sweep_box_planes<T, NOISE, S3>, xl, f[l].p, extp, geo(l), stt, coef[l].p, \
+  launch(grid, threads, lds, C.stream, kb_sweep_box_planes<T, NOISE, S3>, xl, f[l].p, extp, geo(l), stt, coef[l].p, \
          kpar, (int)forward, nplanes, hth, xzf, nk)
       if (stage3) {
         if (noise == NOISE_NONE) MGMC_PLANES(NOISE_NONE, true);
@@ -560,7 +567,7 @@ struct Engine : EngineBase {
     for (int l = 0; l < C.nlevel; ++l)
       if (cur[l]) {
         long long n = (long long)C.L[l]->g.cs * C.K;
-        launch(nblocks(n), 256, 0, C.stream, k_copy<T>, x[l].p, x2[l].p, n);
+        launch(nblocks(n), 256, 0, C.stream, kb_copy<T>, x[l].p, x2[l].p, n);
         cur[l] = 0;
       }
   }
@@ -573,7 +580,7 @@ struct Engine : EngineBase {
     return dim3(nblocks(rows * groups), C.K);
   }
 
-  void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, k_fill<T>, p, n, v); }
+  void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, kb_fill<T>, p, n, v); }
 
   // HIP events around level-0 sweep launches when profiling
   bool prof_begin(int l) {
@@ -637,7 +644,7 @@ struct Engine : EngineBase {
     dim3 grid(nblocks(threads)), blk(256);
     const bool timed = prof_begin(l);
 #define MGMC_SWEEP(DIM, STAR, NOISE) \
-  launch(grid, blk, 0, C.stream, k_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, geo(l), Lv.st, colour, nrows, C.K, \
+  launch(grid, blk, 0, C.stream, kb_sweep<T, DIM, STAR, NOISE>, X(l), f[l].p, extp, geo(l), Lv.st, colour, nrows, C.K, \
          nk)
 #define MGMC_SWEEP_N(DIM, STAR)                                 \
   do {                                                          \
@@ -674,19 +681,19 @@ struct Engine : EngineBase {
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
     dim3 grid(C.m * Lv.nsplit, (C.K + LR_CPB - 1) / LR_CPB);
-    launch(grid, 256, 0, C.stream, k_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p,
+    launch(grid, 256, 0, C.stream, kb_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p,
            Lv.nsplit);
     lr_next = LrSolve{M, noise, extlr, tag, Lv.nsplit};
     if (lr_fused()) return;
     NoiseKey nk = key(tag);
     if (noise == NOISE_NONE)
-      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
              Lv.nsplit);
     else if (noise == NOISE_PHILOX)
-      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
              Lv.nsplit);
     else
-      launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
              nk, Lv.nsplit);
   }
 
@@ -695,13 +702,13 @@ struct Engine : EngineBase {
     if (R.nrows == 0) return;
     dim3 grid(nblocks(R.nrows), (C.K + LR_CPU - 1) / LR_CPU);
     if (!lr_fused()) {
-      launch(grid, 256, 0, C.stream, k_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
+      launch(grid, 256, 0, C.stream, kb_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
              C.K);
       return;
     }
     NoiseKey nk = key(lr_next.tag);
 #define MGMC_LRSU(NOISE)                                                                                        \
-  launch(grid, 256, 0, C.stream, k_lr_solve_update<T, NOISE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p,      \
+  launch(grid, 256, 0, C.stream, kb_lr_solve_update<T, NOISE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p,      \
          R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk, lr_next.nsplit)
     if (lr_next.noise == NOISE_NONE) MGMC_LRSU(NOISE_NONE);
     else if (lr_next.noise == NOISE_PHILOX) MGMC_LRSU(NOISE_PHILOX);
@@ -865,7 +872,7 @@ struct Engine : EngineBase {
     const int threads = ((ngx * (2 * TYc + 3) + 63) / 64) * 64;
     const size_t lds = ((size_t)(4 * TYc + 4) * g.px + (size_t)(2 * TYc + 1) * (g.px >> 1)) * sizeof(T);
     dim3 grid((unsigned)(ntile * nseg * C.K));
-    launch(grid, threads, lds, C.stream, k_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, geo(l), gc, F.st, TYc,
+    launch(grid, threads, lds, C.stream, kb_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, geo(l), gc, F.st, TYc,
            SZc, ntile, nseg);
   }
 
@@ -902,13 +909,13 @@ struct Engine : EngineBase {
       residual_var<T>(l, X(l), f[l].p, r[l].p, geo(l), vcoef(l));
       launch(gq, 256, 0, C.stream, F.g.dim == 3 ? k_restrict<T, 3> : k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
     } else if (F.g.dim == 3) {
-      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 3, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
-      else launch(gr, 256, 0, C.stream, k_residual<T, 3, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
-      launch(gq, 256, 0, C.stream, k_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g);
+      if (F.star) launch(gr, 256, 0, C.stream, kb_residual<T, 3, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      else launch(gr, 256, 0, C.stream, kb_residual<T, 3, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      launch(gq, 256, 0, C.stream, kb_restrict<T, 3>, r[l].p, f[l + 1].p, F.g, Cc.g);
     } else {
-      if (F.star) launch(gr, 256, 0, C.stream, k_residual<T, 2, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
-      else launch(gr, 256, 0, C.stream, k_residual<T, 2, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
-      launch(gq, 256, 0, C.stream, k_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
+      if (F.star) launch(gr, 256, 0, C.stream, kb_residual<T, 2, true>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      else launch(gr, 256, 0, C.stream, kb_residual<T, 2, false>, X(l), f[l].p, r[l].p, geo(l), F.st);
+      launch(gq, 256, 0, C.stream, kb_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
     }
     if (C.m > 0) {
       lr_dot_solve(l, X(l), C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
@@ -932,9 +939,9 @@ struct Engine : EngineBase {
     Level& Cc = *C.L[l + 1];
     dim3 gr = row_grid(F.g, F.g.px >> 3);
     if (F.g.dim == 3)
-      launch(gr, 256, 0, C.stream, k_prolongate<T, 3>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
+      launch(gr, 256, 0, C.stream, kb_prolongate<T, 3>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
     else
-      launch(gr, 256, 0, C.stream, k_prolongate<T, 2>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
+      launch(gr, 256, 0, C.stream, kb_prolongate<T, 2>, X(l), X(l + 1), F.g, Cc.g, (T)alpha);
   }
 
   void coarse(int mode, const double* xi_dev) override {
@@ -957,7 +964,7 @@ struct Engine : EngineBase {
     if (C.tune_coarse_mfma && sh_mfma <= (size_t)C.max_lds) {
       dim3 grid((unsigned)((C.Nc + 15) / 16), (unsigned)((C.K + 15) / 16));
 #define MGMC_COARSE_MFMA(NOISE)                                                                                    \
-  launch(grid, 512, sh_mfma, C.stream, k_coarse_mfma<T, NOISE>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
+  launch(grid, 512, sh_mfma, C.stream, kb_coarse_mfma<T, NOISE>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
          C.Qinv.p, (int)C.Nc, C.K, xi_dev, nk)
       if (mode == NOISE_NONE) MGMC_COARSE_MFMA(NOISE_NONE);
       else if (mode == NOISE_PHILOX) MGMC_COARSE_MFMA(NOISE_PHILOX);
@@ -975,7 +982,7 @@ struct Engine : EngineBase {
     const size_t sh = std::max(2 * (size_t)cpt * C.Nc, (size_t)(threads / 64) * cpt * 64) * sizeof(double);
     dim3 grid((unsigned)((C.Nc + 63) / 64), (unsigned)((C.K + cpt - 1) / cpt));
 #define MGMC_COARSE(NOISE, CPT)                                                                                    \
-  launch(grid, threads, sh, C.stream, k_coarse<T, NOISE, CPT>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
+  launch(grid, threads, sh, C.stream, kb_coarse<T, NOISE, CPT>, X(l), f[l].p, Lv.g.cs, fcs, C.cpidx.p, C.LinvT.p, \
          C.Qinv.p, (int)C.Nc, C.K, xi_dev, nk)
 #define MGMC_COARSE_N(NOISE)                   \
   if (cpt == 8) MGMC_COARSE(NOISE, 8);        \
@@ -1013,8 +1020,11 @@ struct Engine : EngineBase {
   }
 
   // MultigridMCSampler::sample (multigridmc_sampler.cc:109-136)
+  // (rec_mark: the position in the cycle under which a recorder files the
+  // launches that follow, for merging them across instances)
   void sample(int l) {
     if (l == C.nlevel - 1) {
+      rec_mark(0, l, 2);
       if (C.p.coarse_solver == MGMC_COARSE_CHOLESKY) coarse(NOISE_PHILOX, nullptr);
       else
         for (int s = 0; s < C.p.ncoarsesmooth; ++s) {
@@ -1025,10 +1035,13 @@ struct Engine : EngineBase {
     }
     int cyc = l > 0 ? C.p.cycle : 1;
     for (int j = 0; j < cyc; ++j) {
+      rec_mark(0, l, 0);
       pre(l, true);
+      rec_mark(0, l, 1);
       residual_restrict(l);
       zero_x_lazy(l + 1);
       sample(l + 1);
+      rec_mark(1, l, 0);
       prolongate_or_defer(l, C.p.coarse_scaling, C.p.npostsmooth > 0);
       post(l, true);
     }
@@ -1055,6 +1068,7 @@ struct Engine : EngineBase {
   void cycle_sample() override {
     C.tag = 1;
     sample(0);
+    rec_mark(2, 0, 0);
     normalise();
   }
   void cycle_precondition() override {
@@ -1090,7 +1104,7 @@ struct Engine : EngineBase {
 
   void probe(double* obs_dev) override {
     if (C.nprobe == 0 || !obs_dev) return;
-    launch(dim3(C.nprobe, C.K), 64, 0, C.stream, k_observe<T>, X(0), C.L[0]->g.cs, C.obs_ptr.p, C.obs_idx.p,
+    launch(dim3(C.nprobe, C.K), 64, 0, C.stream, kb_observe<T>, X(0), C.L[0]->g.cs, C.obs_ptr.p, C.obs_idx.p,
            C.obs_w.p, C.nprobe, C.K, obs_dev, C.ctr.p);
   }
 
@@ -1467,7 +1481,7 @@ int mgmc_sample(mgmc_handle* h, const double* f, int f_per_chain, double* x) {
   double* dx = stage(h->io2, x, (size_t)N * h->C.K, h->C.stream);
   h->E->set_fine(0, dx, 1);
   h->E->cycle_sample();
-  launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+  launch(1, 64, 0, h->C.stream, kb_advance, h->C.ctr.p);
   h->E->get_fine_x(dx, 1.0);
   HIP_CHECK(hipMemcpyAsync(x, dx, (size_t)N * h->C.K * sizeof(double), hipMemcpyDeviceToHost, h->C.stream));
   sync(h);
@@ -1519,7 +1533,7 @@ int mgmc_set_observables(mgmc_handle* h, int nobs, const int64_t* rowptr, const 
 static void record_cycle(mgmc_handle* h, double* obs_dev) {
   h->E->cycle_sample();
   h->E->probe(obs_dev);
-  launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+  launch(1, 64, 0, h->C.stream, kb_advance, h->C.ctr.p);
 }
 
 static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph) {
@@ -1546,6 +1560,247 @@ static void run_dev(mgmc_handle* h, int nsamples, double* obs_dev, int use_graph
   for (int s = 0; s < nsamples; ++s) HIP_CHECK(hipGraphLaunch(h->gexec, h->C.stream));
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent instances (mgmc_batch_*): one cycle of every
+// instance is recorded (the engines issue their launches into a Recorder
+// instead of the device), the records are ordered by their position in the
+// V-cycle - (down, level, stage), the coarsest level, (up, level), the
+// observables - and the launches of one kernel body at the same position
+// become one k_multi launch whose blocks are the instances' blocks side by
+// side: the uneven lattices of a batch are columns of one launch per level
+// phase. The merged launches of a cycle are captured in one graph.
+// ---------------------------------------------------------------------------
+struct mgmc_batch {
+  std::vector<mgmc_handle*> hs;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  int device = 0;
+  int persistent = 0;  // workgroups per CU of a merged launch (0: one workgroup per block)
+  struct Group {
+    MultiLauncher launcher = nullptr;
+    const void* multi = nullptr;
+    dim3 block;
+    size_t shmem = 0, off = 0;
+    int n = 0;
+    uint32_t total = 0, grid = 0;
+  };
+  std::vector<Group> groups;
+  DevBuf<char> items;
+  int64_t nrec = 0;  // launches per cycle before merging
+  bool planned = false;
+  std::vector<double*> obs;
+  std::vector<int> fshared;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ~mgmc_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void drop_graph() {
+    if (gexec) HIP_CHECK(hipGraphExecDestroy(gexec));
+    if (graph) HIP_CHECK(hipGraphDestroy(graph));
+    gexec = nullptr;
+    graph = nullptr;
+  }
+};
+
+static void batch_plan(mgmc_batch* b, double* const* obs) {
+  std::vector<LaunchRec> all;
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    Recorder R;
+    R.instance = (int)i;
+    g_rec = &R;
+    try {
+      record_cycle(b->hs[i], obs ? obs[i] : nullptr);
+    } catch (...) {
+      g_rec = nullptr;
+      throw;
+    }
+    g_rec = nullptr;
+    for (LaunchRec& r : R.recs) all.push_back(std::move(r));
+  }
+  b->nrec = (int64_t)all.size();
+  // cycle order: down the levels (pre-smoothing, residual + restriction,
+  // the coarsest level), up the levels (prolongation, post-smoothing), then
+  // the observables; each instance's own order is kept (stable sort)
+  auto pos = [](const LaunchRec& r) { return std::make_tuple(r.dir, r.dir == 1 ? -r.level : r.level, r.stage, r.seq); };
+  std::stable_sort(all.begin(), all.end(), [&](const LaunchRec& a, const LaunchRec& c) { return pos(a) < pos(c); });
+  std::vector<char> host;
+  std::vector<mgmc_batch::Group> groups;
+  size_t i = 0;
+  while (i < all.size()) {
+    size_t e = i;
+    while (e < all.size() && pos(all[e]) == pos(all[i])) ++e;
+    // within one position: one group per body (and per workgroup size for
+    // bodies whose results depend on it), in order of first appearance
+    std::vector<bool> used(e - i, false);
+    for (size_t a = i; a < e; ++a) {
+      if (used[a - i]) continue;
+      mgmc_batch::Group g;
+      g.multi = all[a].multi;
+      g.launcher = all[a].launcher;
+      const size_t align = std::max<size_t>(16, all[a].item_align);
+      g.off = (host.size() + align - 1) / align * align;
+      host.resize(g.off);
+      uint32_t start = 0;
+      for (size_t c = a; c < e; ++c) {
+        const LaunchRec& r = all[c];
+        if (used[c - i] || r.multi != g.multi || (!r.tolerant && r.block.x != all[a].block.x)) continue;
+        used[c - i] = true;
+        const size_t at = host.size();
+        host.insert(host.end(), r.item.begin(), r.item.end());
+        std::memcpy(host.data() + at, &start, sizeof(uint32_t));  // MultiItem::start
+        start += (r.grid.x * r.grid.y + 7) / 8 * 8;
+        g.block.x = std::max(g.block.x, r.block.x);
+        g.shmem = std::max(g.shmem, r.shmem);
+        ++g.n;
+      }
+      g.total = start;
+      g.grid = start;
+      groups.push_back(g);
+    }
+    i = e;
+  }
+  int max_lds = b->hs[0]->C.max_lds;
+  for (mgmc_batch::Group& g : groups) {
+#ifndef MGMC_EMULATION
+    if (g.shmem > 65536) {
+      if ((int)g.shmem > max_lds) throw Error(MGMC_ERR_UNSUPPORTED, "merged launch needs more LDS than the device has");
+      HIP_CHECK(hipFuncSetAttribute(g.multi, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    }
+    if (b->persistent > 0) {
+      int per_cu = 0, ncu = 0;
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.multi, (int)g.block.x, g.shmem));
+      HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+      const uint32_t cap = (uint32_t)std::max(1, std::min(per_cu, b->persistent) * ncu);
+      g.grid = std::min(g.total, cap);
+    }
+#else
+    // emulation: three "CUs", so that the workgroups loop over many blocks
+    if (b->persistent > 0) g.grid = std::min<uint32_t>(g.total, (uint32_t)b->persistent * 3);
+#endif
+  }
+  b->items.alloc(host.size() ? host.size() : 1);
+  if (!host.empty()) HIP_CHECK(hipMemcpy(b->items.p, host.data(), host.size(), hipMemcpyHostToDevice));
+  b->groups = std::move(groups);
+  b->drop_graph();
+  b->obs.assign(b->hs.size(), nullptr);
+  if (obs)
+    for (size_t q = 0; q < b->hs.size(); ++q) b->obs[q] = obs[q];
+  b->fshared.clear();
+  for (mgmc_handle* h : b->hs) b->fshared.push_back(h->E->f_shared() ? 1 : 0);
+  b->planned = true;
+}
+
+static void batch_cycle(mgmc_batch* b) {
+  for (const mgmc_batch::Group& g : b->groups)
+    g.launcher(dim3(g.grid), g.block, g.shmem, b->stream, b->items.p + g.off, g.n, g.total);
+}
+
+extern "C" {
+
+int mgmc_batch_create(int n, mgmc_handle* const* handles, mgmc_batch** out) {
+  API_BEGIN
+  if (n < 1 || !handles || !out) throw Error(MGMC_ERR_ARG, "mgmc_batch_create: n >= 1 handles and an output pointer");
+  auto b = std::make_unique<mgmc_batch>();
+  for (int i = 0; i < n; ++i) {
+    mgmc_handle* h = handles[i];
+    if (!h) throw Error(MGMC_ERR_ARG, "mgmc_batch_create: null handle");
+    for (mgmc_handle* q : b->hs)
+      if (q == h) throw Error(MGMC_ERR_ARG, "mgmc_batch_create: a handle appears twice");
+    if (h->C.device != handles[0]->C.device) throw Error(MGMC_ERR_ARG, "mgmc_batch_create: handles on different devices");
+    if (h->C.p.cycle != 1 && h->C.nlevel > 2)
+      throw Error(MGMC_ERR_UNSUPPORTED, "mgmc_batch_create: batches run V-cycles (cycle = 1)");
+    if (h->C.ordering != MGMC_ORDER_MULTICOLOUR)
+      throw Error(MGMC_ERR_UNSUPPORTED, "mgmc_batch_create: batches run the multicolour order");
+    b->hs.push_back(h);
+  }
+  b->device = handles[0]->C.device;
+  HIP_CHECK(hipSetDevice(b->device));
+  HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  // a first recording checks that every instance's cycle can be merged
+  batch_plan(b.get(), nullptr);
+  *out = b.release();
+  API_END
+}
+
+int mgmc_batch_destroy(mgmc_batch* b) {
+  API_BEGIN
+  if (b) HIP_CHECK(hipStreamSynchronize(b->stream));
+  delete b;
+  API_END
+}
+
+int mgmc_batch_set_persistent(mgmc_batch* b, int workgroups_per_cu) {
+  API_BEGIN
+  if (workgroups_per_cu < 0) throw Error(MGMC_ERR_ARG, "workgroups_per_cu < 0");
+  b->persistent = workgroups_per_cu;
+  b->planned = false;
+  API_END
+}
+
+int mgmc_batch_stats(const mgmc_batch* b, int64_t* launches_per_cycle, int64_t* separate_launches_per_cycle) {
+  API_BEGIN
+  if (launches_per_cycle) *launches_per_cycle = (int64_t)b->groups.size();
+  if (separate_launches_per_cycle) *separate_launches_per_cycle = b->nrec;
+  API_END
+}
+
+int mgmc_batch_run_dev(mgmc_batch* b, int nsamples, double* const* obs_dev, int use_graph) {
+  API_BEGIN
+  if (nsamples < 0) throw Error(MGMC_ERR_ARG, "nsamples < 0");
+  HIP_CHECK(hipSetDevice(b->device));
+  for (mgmc_handle* h : b->hs) HIP_CHECK(hipStreamSynchronize(h->C.stream));
+  bool same = b->planned;
+  for (size_t i = 0; same && i < b->hs.size(); ++i) {
+    double* o = obs_dev && b->hs[i]->C.nprobe ? obs_dev[i] : nullptr;
+    same = b->obs[i] == o && b->fshared[i] == (b->hs[i]->E->f_shared() ? 1 : 0);
+  }
+  if (!same) {
+    std::vector<double*> o(b->hs.size(), nullptr);
+    for (size_t i = 0; i < b->hs.size(); ++i) o[i] = obs_dev && b->hs[i]->C.nprobe ? obs_dev[i] : nullptr;
+    batch_plan(b, o.data());
+  }
+  // observable rows start at 0 for this run (outside any captured graph)
+  for (mgmc_handle* h : b->hs)
+    HIP_CHECK(hipMemsetAsync(h->C.ctr.p + 1, 0, sizeof(uint32_t), b->stream));
+#ifdef MGMC_EMULATION
+  use_graph = 0;
+#endif
+  if (!use_graph) {
+    for (int s = 0; s < nsamples; ++s) batch_cycle(b);
+  } else if (nsamples > 0) {
+    if (!b->gexec) {
+      HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        batch_cycle(b);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(b->stream, &b->graph));
+      HIP_CHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsamples; ++s) HIP_CHECK(hipGraphLaunch(b->gexec, b->stream));
+  }
+  HIP_CHECK(hipGetLastError());
+  // later work on the instances' own streams waits for the batch
+  HIP_CHECK(hipEventRecord(b->done, b->stream));
+  for (mgmc_handle* h : b->hs) HIP_CHECK(hipStreamWaitEvent(h->C.stream, b->done, 0));
+  API_END
+}
+
+}  // extern "C"
+
+extern "C" {
+
 int mgmc_moments(mgmc_handle* h, int nsamples, double* mean, double* variance) {
   API_BEGIN
   if (nsamples < 1) throw Error(MGMC_ERR_ARG, "nsamples < 1");
@@ -1561,7 +1816,7 @@ int mgmc_moments(mgmc_handle* h, int nsamples, double* mean, double* variance) {
   for (int s = 0; s < nsamples; ++s) {
     h->E->cycle_sample();
     h->E->accumulate_moments(m1.p, m2.p);
-    launch(1, 64, 0, h->C.stream, k_advance, h->C.ctr.p);
+    launch(1, 64, 0, h->C.stream, kb_advance, h->C.ctr.p);
   }
   const double scale = 1.0 / ((double)nsamples * h->C.K);
   std::vector<double> e1((size_t)N), e2((size_t)N);

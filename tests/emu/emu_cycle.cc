@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "../../include/mgmc_amd.h"
@@ -28,14 +30,22 @@ struct Case {
   bool variable;
 };
 
-void run(const Case& c) {
-  std::printf("case %s\n", c.name);
-  std::fflush(stdout);
+// The problem of a case: its measurement CSR, variances and diffusion
+// field, kept alive while handles are created from it.
+struct ProblemData {
   int64_t N = 1;
-  for (int d = 0; d < c.dim; ++d) N *= c.n[d] - 1;
+  std::vector<int64_t> rp, col;
+  std::vector<double> bval, sigma, diffusion;
+  mgmc_problem p{};
+  mgmc_params par{};
+};
+
+void build(const Case& c, ProblemData& d) {
+  int64_t N = 1;
+  for (int dd = 0; dd < c.dim; ++dd) N *= c.n[dd] - 1;
+  d.N = N;
   // measurements: balls of radius 0.2 at seeded centres
   std::vector<std::vector<std::pair<int64_t, double>>> rows((size_t)N);
-  std::vector<double> sigma;
   for (int k = 0; k < c.m; ++k) {
     double x0[3] = {0.3 + 0.17 * k, 0.6 - 0.11 * k, 0.45};
     std::vector<int64_t> idx((size_t)N);
@@ -43,34 +53,33 @@ void run(const Case& c) {
     int64_t nnz = 0;
     check(mgmc_measurement_vector(c.dim, c.n, x0, 0.2, N, idx.data(), val.data(), &nnz), "measurement_vector");
     for (int64_t e = 0; e < nnz; ++e) rows[(size_t)idx[e]].push_back({k, val[e]});
-    sigma.push_back(1e-3 * (1.0 + k));
+    d.sigma.push_back(1e-3 * (1.0 + k));
   }
-  std::vector<int64_t> rp(1, 0), col;
-  std::vector<double> bval;
+  d.rp.assign(1, 0);
   for (auto& r : rows) {
     for (auto& e : r) {
-      col.push_back(e.first);
-      bval.push_back(e.second);
+      d.col.push_back(e.first);
+      d.bval.push_back(e.second);
     }
-    rp.push_back((int64_t)col.size());
+    d.rp.push_back((int64_t)d.col.size());
   }
   int64_t ncell = 1;
-  for (int d = 0; d < c.dim; ++d) ncell *= c.n[d];
-  std::vector<double> diffusion((size_t)ncell);
-  for (int64_t i = 0; i < ncell; ++i) diffusion[(size_t)i] = 1.0 + 9.0 * (double)((i * 7919) % 13) / 12.0;
-  mgmc_problem p{};
+  for (int dd = 0; dd < c.dim; ++dd) ncell *= c.n[dd];
+  d.diffusion.resize((size_t)ncell);
+  for (int64_t i = 0; i < ncell; ++i) d.diffusion[(size_t)i] = 1.0 + 9.0 * (double)((i * 7919) % 13) / 12.0;
+  mgmc_problem& p = d.p;
   p.dim = c.dim;
-  for (int d = 0; d < 3; ++d) p.n[d] = c.n[d];
+  for (int dd = 0; dd < 3; ++dd) p.n[dd] = c.n[dd];
   p.kappa_sq = 10.0;
   p.m = c.m;
   if (c.m > 0) {
-    p.B_rowptr = rp.data();
-    p.B_col = col.data();
-    p.B_val = bval.data();
-    p.sigma = sigma.data();
+    p.B_rowptr = d.rp.data();
+    p.B_col = d.col.data();
+    p.B_val = d.bval.data();
+    p.sigma = d.sigma.data();
   }
-  if (c.variable) p.diffusion = diffusion.data();
-  mgmc_params par{};
+  if (c.variable) p.diffusion = d.diffusion.data();
+  mgmc_params& par = d.par;
   par.nlevel = c.nlevel;
   par.smoother = MGMC_SSOR;
   par.npresmooth = 1;
@@ -83,6 +92,16 @@ void run(const Case& c) {
   par.precision = c.precision;
   par.nchains = c.K;
   par.seed = 7;
+}
+
+void run(const Case& c) {
+  std::printf("case %s\n", c.name);
+  std::fflush(stdout);
+  ProblemData d;
+  build(c, d);
+  const int64_t N = d.N;
+  mgmc_problem& p = d.p;
+  mgmc_params& par = d.par;
   mgmc_handle* h = nullptr;
   check(mgmc_create(&p, &par, &h), "mgmc_create");
   if (c.ordering != MGMC_ORDER_MULTICOLOUR) check(mgmc_set_ordering(h, c.ordering), "mgmc_set_ordering");
@@ -119,6 +138,88 @@ void run(const Case& c) {
   check(mgmc_destroy(h), "mgmc_destroy");
 }
 
+// A batch of instances of different sizes, precisions and chain counts:
+// the merged launches (mgmc_batch_run_dev) must give every instance the
+// observables and final state of its own run, bit for bit.
+void batch_case() {
+  std::printf("case batch of 4 instances (merged launches)\n");
+  std::fflush(stdout);
+  const Case cs[] = {
+      {"16^3 fp32", 3, {16, 16, 16}, 3, 3, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 2, false},
+      {"12x16x8 fp64", 3, {12, 16, 8}, 2, 2, MGMC_FP64, MGMC_ORDER_MULTICOLOUR, 1, false},
+      {"24^3 fp32", 3, {24, 24, 24}, 2, 3, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 3, false},
+      {"16^3 fp32 no measurements", 3, {16, 16, 16}, 0, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 1, false},
+  };
+  const int n = 4, ncycles = 2;
+  ProblemData d[n];
+  mgmc_handle* own[n];
+  mgmc_handle* bat[n];
+  std::vector<double> obs_own[n], obs_bat[n];
+  double* ptrs[n];
+  for (int i = 0; i < n; ++i) {
+    build(cs[i], d[i]);
+    d[i].par.seed = 100 + i;
+    check(mgmc_create(&d[i].p, &d[i].par, &own[i]), "mgmc_create");
+    check(mgmc_create(&d[i].p, &d[i].par, &bat[i]), "mgmc_create");
+    const int64_t N = d[i].N;
+    std::vector<double> f((size_t)N), x((size_t)(N * cs[i].K), 0.0);
+    for (int64_t v = 0; v < N; ++v) f[(size_t)v] = std::cos(0.29 * (double)v + i);
+    const int64_t probes[3] = {1, N / 3, N - 2};
+    for (mgmc_handle* h : {own[i], bat[i]}) {
+      check(mgmc_set_rhs(h, f.data(), 0), "mgmc_set_rhs");
+      check(mgmc_set_state(h, x.data()), "mgmc_set_state");
+      check(mgmc_set_probes(h, 3, probes), "mgmc_set_probes");
+    }
+    obs_own[i].assign((size_t)(ncycles * cs[i].K * 3), 0.0);
+    obs_bat[i].assign((size_t)(ncycles * cs[i].K * 3), -1.0);
+    // emulated device memory is host memory
+    check(mgmc_run_dev(own[i], ncycles, obs_own[i].data(), 0), "mgmc_run_dev");
+    ptrs[i] = obs_bat[i].data();
+  }
+  mgmc_batch* b = nullptr;
+  check(mgmc_batch_create(n, bat, &b), "mgmc_batch_create");
+  check(mgmc_batch_run_dev(b, ncycles, ptrs, 0), "mgmc_batch_run_dev");
+  int64_t merged = 0, separate = 0;
+  check(mgmc_batch_stats(b, &merged, &separate), "mgmc_batch_stats");
+  std::printf("  %lld launches per cycle merged, %lld separate\n", (long long)merged, (long long)separate);
+  if (!(merged < separate)) {
+    std::printf("FAIL batch: no launch was merged\n");
+    ++failures;
+  }
+  for (int i = 0; i < n; ++i) {
+    if (std::memcmp(obs_own[i].data(), obs_bat[i].data(), obs_own[i].size() * sizeof(double)) != 0) {
+      std::printf("FAIL batch: observables of instance %d differ\n", i);
+      ++failures;
+    }
+    const size_t nx = (size_t)(d[i].N * cs[i].K);
+    std::vector<double> xo(nx), xb(nx);
+    check(mgmc_get_state(own[i], xo.data()), "mgmc_get_state");
+    check(mgmc_get_state(bat[i], xb.data()), "mgmc_get_state");
+    if (std::memcmp(xo.data(), xb.data(), nx * sizeof(double)) != 0) {
+      std::printf("FAIL batch: final state of instance %d differs\n", i);
+      ++failures;
+    }
+  }
+  // the persistent form (workgroups looping over the blocks of every
+  // instance) continues the same chains: compare with two more own cycles
+  check(mgmc_batch_set_persistent(b, 1), "mgmc_batch_set_persistent");
+  for (int i = 0; i < n; ++i) {
+    check(mgmc_run_dev(own[i], ncycles, obs_own[i].data(), 0), "mgmc_run_dev");
+    std::fill(obs_bat[i].begin(), obs_bat[i].end(), -1.0);
+  }
+  check(mgmc_batch_run_dev(b, ncycles, ptrs, 0), "mgmc_batch_run_dev persistent");
+  for (int i = 0; i < n; ++i)
+    if (std::memcmp(obs_own[i].data(), obs_bat[i].data(), obs_own[i].size() * sizeof(double)) != 0) {
+      std::printf("FAIL batch: persistent observables of instance %d differ\n", i);
+      ++failures;
+    }
+  check(mgmc_batch_destroy(b), "mgmc_batch_destroy");
+  for (int i = 0; i < n; ++i) {
+    check(mgmc_destroy(own[i]), "mgmc_destroy");
+    check(mgmc_destroy(bat[i]), "mgmc_destroy");
+  }
+}
+
 }  // namespace
 
 int main() {
@@ -131,6 +232,7 @@ int main() {
       {"3d 8^3, variable diffusion, fp64", 3, {8, 8, 8}, 2, 2, MGMC_FP64, MGMC_ORDER_MULTICOLOUR, 2, true},
   };
   for (const Case& c : cases) run(c);
+  batch_case();
   if (failures) return 1;
   std::printf("emu_cycle ok\n");
   return 0;

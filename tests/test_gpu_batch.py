@@ -1,8 +1,11 @@
 """A mixed batch of 16 posterior sampling problems on lattices drawn from
-{32^3, 48^3, 64^3, 96^3, 128^3} with 8-32 ball measurements each, run
-concurrently (one stream and one captured cycle per instance): every
-instance equals its own sequential run, and its observables have the exact
-posterior means and variances from FP64 multigrid solves."""
+{32^3, 48^3, 64^3, 96^3, 128^3} with 8-32 ball measurements each, run as
+one batch whose launches are merged across instances (one launch per kernel
+and point of the V-cycle, the lattices side by side as columns, one captured
+graph per cycle): every instance equals its own sequential run, and its
+observables have the exact posterior means and variances from FP64
+multigrid solves. The persistent form of the merged launches and the
+one-stream-per-instance form give the same samples."""
 import numpy as np
 import pytest
 
@@ -13,11 +16,11 @@ from multigridmc_amd.batch import SamplerBatch
 pytestmark = pytest.mark.gpu
 
 
-def _instances(seed=2024, count=16, nchains=4):
+def _instances(seed=2024, count=16, nchains=4, sizes=(32, 48, 64, 96, 128)):
     rng = np.random.default_rng(seed)
     out = []
     for k in range(count):
-        n = int(rng.choice([32, 48, 64, 96, 128]))
+        n = int(rng.choice(sizes))
         lat = mg.Lattice(n, n, n)
         m = int(rng.integers(8, 33))
         centres, y, var = mg.seeded_ball_measurements(lat, m, 0.05, 1e-4, seed + k)
@@ -30,23 +33,29 @@ def _instances(seed=2024, count=16, nchains=4):
     return out
 
 
+def _sequential(i, nwarm, nsamples):
+    s = mg.MultigridMCSampler(i["op"], i["params"], nchains=i["nchains"], precision=i.get("precision", "fp32"),
+                              seed=i["seed"])
+    s.fix_rhs(i["f"])
+    s.set_state(np.zeros((i["nchains"], i["op"].lattice.Nvertex)))
+    s.set_observables(i["observables"])
+    s.run(nwarm, record=False)
+    return s.run(nsamples)
+
+
 def test_mixed_batch_equals_sequential_and_matches_posterior():
     inst = _instances()
     sizes = sorted({i["op"].lattice.n[0] for i in inst})
     assert len(sizes) >= 3
     batch = SamplerBatch(inst)
+    merged, separate = batch.stats()
+    print(f"mixed batch of 16: {merged} launches per cycle merged, {separate} separate", flush=True)
+    assert merged < separate / 8
     batch.run(30, record=False)
     z = batch.run(300)
     # the same instances one after the other (same seeds): identical samples
     for k in (0, 5, 15):
-        i = inst[k]
-        s = mg.MultigridMCSampler(i["op"], i["params"], nchains=i["nchains"], precision="fp32", seed=i["seed"])
-        s.fix_rhs(i["f"])
-        s.set_state(np.zeros((i["nchains"], i["op"].lattice.Nvertex)))
-        s.set_observables(i["observables"])
-        s.run(30, record=False)
-        zs = s.run(300)
-        assert np.array_equal(zs, z[k]), k
+        assert np.array_equal(_sequential(inst[k], 30, 300), z[k]), k
     # posterior moments of the observables, within Monte Carlo errors
     for k, i in enumerate(inst):
         means, cov = ms.observed_moments(i["op"], np.zeros(i["op"].lattice.Nvertex), i["y"], i["observables"],
@@ -57,3 +66,23 @@ def test_mixed_batch_equals_sequential_and_matches_posterior():
             se = np.sqrt(cov[p, p] * tau / series.size)
             assert abs(series.mean() - means[p]) < 5 * se, (k, p)
             assert abs(series.var() / cov[p, p] - 1.0) < 6 * np.sqrt(2 * tau / series.size) + 0.02, (k, p)
+
+
+def test_batch_forms_agree():
+    """Merged launches replayed as a graph, issued eagerly, in the persistent
+    form (one and two workgroups per CU), and one stream per instance: the
+    same samples for FP32 and FP64 instances with different chain counts."""
+    inst = _instances(seed=99, count=6, nchains=3, sizes=(32, 48, 64))
+    inst[1]["precision"] = "fp64"
+    inst[2]["nchains"] = 1
+    inst[4]["precision"] = "fp64"
+    inst[4]["nchains"] = 5
+    ref = [_sequential(i, 3, 20) for i in inst]
+    for mode, persistent, graph in (("merged", 0, True), ("merged", 0, False), ("merged", 1, True),
+                                    ("merged", 2, True), ("streams", 0, True)):
+        b = SamplerBatch(inst, mode=mode, persistent=persistent)
+        b.run(3, record=False, use_graph=graph)
+        z = b.run(20, use_graph=graph)
+        for k in range(len(inst)):
+            assert np.array_equal(ref[k], z[k]), (mode, persistent, graph, k)
+        b.close()
