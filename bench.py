@@ -169,17 +169,17 @@ def mixed_batch(args):
     chains = sum(i["nchains"] for i in inst)
     out = {"workload": "mixed batch of 16 instances, lattices " +
                        ",".join(str(i["op"].lattice.n[0]) for i in inst) + ", 4 FP32 chains each",
-           "chains": chains, "steps": args.steps, "warmup": args.warmup}
-    for name, mode, persistent in (("merged", "merged", 0), ("merged_persistent", "merged", 1),
+           "chains": chains, "steps": args.steps, "warmup": args.warmup, "graph": not args.no_graph}
+    for name, mode, persistent in (("merged", "merged", 0), ("merged_persistent", "merged", args.persistent),
                                    ("streams", "streams", 0)):
         b = SamplerBatch(inst, mode=mode, persistent=persistent)
         bufs = [torch.empty((max(args.steps, args.warmup), s.nchains, 1), dtype=torch.float64, device="cuda")
                 for s in b.samplers]
         torch.cuda.synchronize()
-        b.run_dev(args.warmup, bufs)
+        b.run_dev(args.warmup, bufs, not args.no_graph)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        b.run_dev(args.steps, bufs)
+        b.run_dev(args.steps, bufs, not args.no_graph)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         st = b.stats()
@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--mixed-batch", action="store_true",
                     help="only time the mixed batch of 16 instances (32^3..128^3, 8-32 balls, 4 FP32 chains each): "
                          "merged launches (plain and persistent) against one stream per instance")
+    ap.add_argument("--persistent", type=int, default=1,
+                    help="workgroups per CU of the persistent merged launches (--mixed-batch)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()

@@ -598,11 +598,10 @@ __global__ void __launch_bounds__(256) k_sweep_lex(T* __restrict__ x, const T* _
 // Residual:   M = Sigma^{-1}, a = 0, zc = -1 -> subtract B Sigma^{-1} B^T x.
 // ---------------------------------------------------------------------------
 constexpr int MMAX = 128;
-// z[c][k] = (B^T x_c)_k for the LR_CPB chains c0 .. c0 + LR_CPB - 1 of a
+// z[c][k] = (B^T x_c)_k for the LR_CPB (1, 2, 4) chains c0 .. c0 + LR_CPB - 1 of a
 // block (grid: columns x chain groups): the column's indices and values are
 // read once and the chains' x loads are independent.
-constexpr int LR_CPB = 1;
-template <typename T>
+template <typename T, int LR_CPB>
 __device__ __forceinline__ void b_lr_dot(const Blk bk, const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
                                                 const long long* __restrict__ bidx, const double* __restrict__ bval,
                                                 int m, int K, double* __restrict__ z, int nsplit) {
@@ -659,14 +658,14 @@ __device__ __forceinline__ void b_lr_dot(const Blk bk, const T* __restrict__ x, 
     z[((long long)(c0 + q) * m + kcol) * nsplit + part] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
   }
 }
-template <typename T>
+template <typename T, int LR_CPB>
 __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long long cs, const int32_t* __restrict__ bptr,
                                                 const long long* __restrict__ bidx, const double* __restrict__ bval,
                                                 int m, int K, double* __restrict__ z, int nsplit) {
-  b_lr_dot<T>(Blk::hw(), x, cs, bptr, bidx, bval, m, K, z, nsplit);
+  b_lr_dot<T, LR_CPB>(Blk::hw(), x, cs, bptr, bidx, bval, m, K, z, nsplit);
 }
-template <typename T>
-constexpr Batchable<b_lr_dot<T>, 256, false, k_lr_dot<T>> kb_lr_dot{};
+template <typename T, int LR_CPB>
+constexpr Batchable<b_lr_dot<T, LR_CPB>, 256, false, k_lr_dot<T, LR_CPB>> kb_lr_dot{};
 
 // one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c)
 template <int NOISE>
@@ -705,17 +704,17 @@ __global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z,
 template <int NOISE>
 constexpr Batchable<b_lr_solve<NOISE>, MMAX, false, k_lr_solve<NOISE>> kb_lr_solve{};
 
-// x_c[rows[r]] += (R u_c)_r for the LR_CPU chains of a block (grid: row
+// x_c[rows[r]] += (R u_c)_r for the CPU chains of a block (grid: row
 // blocks x chain groups); a row's entries are read once for all its chains.
 constexpr int LR_CPU = 8;
-template <typename T>
+template <typename T, int CPU>
 __device__ __forceinline__ void b_lr_update(const Blk bk, T* __restrict__ x, long long cs, const long long* __restrict__ rows,
                                                    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                    const double* __restrict__ val, int nrows,
                                                    const double* __restrict__ u, int m, int K) {
-  __shared__ double us[LR_CPU][MMAX];
-  const int c0 = bk.y * LR_CPU;
-  const int nc = K - c0 < LR_CPU ? K - c0 : LR_CPU;
+  __shared__ double us[CPU][MMAX];
+  const int c0 = bk.y * CPU;
+  const int nc = K - c0 < CPU ? K - c0 : CPU;
   const int tid = threadIdx.x;
   const int r = bk.x * blockDim.x + tid;
   const bool ok = r < nrows;
@@ -728,35 +727,35 @@ __device__ __forceinline__ void b_lr_update(const Blk bk, T* __restrict__ x, lon
     e0 = ptr[r];
     e1 = ptr[r + 1];
   }
-  T xv[LR_CPU];
+  T xv[CPU];
 #pragma unroll
-  for (int q = 0; q < LR_CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
+  for (int q = 0; q < CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
   for (int e = tid; e < nc * m; e += blockDim.x) us[e / m][e % m] = u[(long long)c0 * m + e];
   __syncthreads();
   if (!ok) return;
-  double s[LR_CPU];
+  double s[CPU];
 #pragma unroll
-  for (int q = 0; q < LR_CPU; ++q) s[q] = 0.0;
+  for (int q = 0; q < CPU; ++q) s[q] = 0.0;
   for (int e = e0; e < e1; ++e) {
     const double v = val[e];
     const int cl = col[e];
 #pragma unroll
-    for (int q = 0; q < LR_CPU; ++q) s[q] += v * us[q][cl];
+    for (int q = 0; q < CPU; ++q) s[q] += v * us[q][cl];
   }
   T* p = x + (long long)c0 * cs + rr;
 #pragma unroll
-  for (int q = 0; q < LR_CPU; ++q)
+  for (int q = 0; q < CPU; ++q)
     if (q < nc) p[q * cs] = (T)((double)xv[q] + s[q]);
 }
-template <typename T>
+template <typename T, int CPU>
 __global__ void __launch_bounds__(256) k_lr_update(T* __restrict__ x, long long cs, const long long* __restrict__ rows,
                                                    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                    const double* __restrict__ val, int nrows,
                                                    const double* __restrict__ u, int m, int K) {
-  b_lr_update<T>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, u, m, K);
+  b_lr_update<T, CPU>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, u, m, K);
 }
-template <typename T>
-constexpr Batchable<b_lr_update<T>, 256, false, k_lr_update<T>> kb_lr_update{};
+template <typename T, int CPU>
+constexpr Batchable<b_lr_update<T, CPU>, 256, false, k_lr_update<T, CPU>> kb_lr_update{};
 
 // k_lr_solve and k_lr_update in one launch: every block first forms u for
 // its LR_CPU chains (u_c = M (zc z_c + sqrt(Sigma) xi'_c), the same sums in

@@ -205,6 +205,8 @@ struct Common {
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
   int tune_coarse_mfma = 1;   // coarsest-level products on the FP64 matrix cores
   int tune_rr_threads = 512;
+  int tune_lr_dot_chains = 1;     // chains per workgroup of k_lr_dot (1, 2, 4)
+  int tune_lr_update_chains = 8;  // chains per thread of k_lr_update (2, 4, 8, 16)
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
   int tune_march_blocks = 512; // ... workgroups they aim for
   int tune_march_min_planes = 4; // ... fewest planes per z-segment
@@ -584,7 +586,7 @@ struct Engine : EngineBase {
 
   // HIP events around level-0 sweep launches when profiling
   bool prof_begin(int l) {
-    if (!(C.prof && l == 0)) return false;
+    if (!(C.prof && l == 0) || g_rec) return false;
     if (C.ev_used + 2 > C.ev.size()) {
       for (int q = 0; q < 64; ++q) {
         hipEvent_t e;
@@ -680,9 +682,15 @@ struct Engine : EngineBase {
   bool lr_fused() const { return C.K <= LR_CPU; }
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
-    dim3 grid(C.m * Lv.nsplit, (C.K + LR_CPB - 1) / LR_CPB);
-    launch(grid, 256, 0, C.stream, kb_lr_dot<T>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p,
-           Lv.nsplit);
+    const int cpb = C.tune_lr_dot_chains;
+    dim3 grid(C.m * Lv.nsplit, (C.K + cpb - 1) / cpb);
+#define MGMC_LRDOT(CPB) \
+  launch(grid, 256, 0, C.stream, kb_lr_dot<T, CPB>, xp, Lv.g.cs, Lv.bptr.p, Lv.bidx.p, Lv.bval.p, C.m, C.K, C.z.p, \
+         Lv.nsplit)
+    if (cpb == 4) MGMC_LRDOT(4);
+    else if (cpb == 2) MGMC_LRDOT(2);
+    else MGMC_LRDOT(1);
+#undef MGMC_LRDOT
     lr_next = LrSolve{M, noise, extlr, tag, Lv.nsplit};
     if (lr_fused()) return;
     NoiseKey nk = key(tag);
@@ -702,8 +710,16 @@ struct Engine : EngineBase {
     if (R.nrows == 0) return;
     dim3 grid(nblocks(R.nrows), (C.K + LR_CPU - 1) / LR_CPU);
     if (!lr_fused()) {
-      launch(grid, 256, 0, C.stream, kb_lr_update<T>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m,
-             C.K);
+      const int cpu = C.tune_lr_update_chains;
+      dim3 gu(nblocks(R.nrows), (C.K + cpu - 1) / cpu);
+#define MGMC_LRUP(CPU) \
+  launch(gu, 256, 0, C.stream, kb_lr_update<T, CPU>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p, R.nrows, C.u.p, C.m, \
+         C.K)
+      if (cpu == 16) MGMC_LRUP(16);
+      else if (cpu == 4) MGMC_LRUP(4);
+      else if (cpu == 2) MGMC_LRUP(2);
+      else MGMC_LRUP(8);
+#undef MGMC_LRUP
       return;
     }
     NoiseKey nk = key(lr_next.tag);
@@ -1148,16 +1164,16 @@ struct Engine : EngineBase {
       else launch(gr, 256, 0, C.stream, k_residual<double, 2, false>, x64, b64, r64, F.g, F.st);
     }
     if (C.m > 0) {
-      dim3 grid(C.m, (C.K + LR_CPB - 1) / LR_CPB);
+      dim3 grid(C.m, C.K);
       NoiseKey nk = key(0);
       grid.x *= F.nsplit;
-      launch(grid, 256, 0, C.stream, k_lr_dot<double>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p,
+      launch(grid, 256, 0, C.stream, k_lr_dot<double, 1>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p,
              F.nsplit);
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, C.sigma_inv_mat.p, C.sqrt_sigma.p, -1.0, nullptr,
              C.u.p, C.m, nk, F.nsplit);
       dim3 g2(nblocks(F.Brows.nrows), (C.K + LR_CPU - 1) / LR_CPU);
       if (F.Brows.nrows)
-        launch(g2, 256, 0, C.stream, k_lr_update<double>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
+        launch(g2, 256, 0, C.stream, k_lr_update<double, LR_CPU>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
                F.Brows.val.p, F.Brows.nrows, C.u.p, C.m, C.K);
     }
   }
@@ -1189,6 +1205,7 @@ struct mgmc_handle {
   double* graph_obs = nullptr;
   int graph_valid = 0;
   bool graph_fshared = false;  // level-0 f layout the graph was captured with
+  uint64_t generation = 0;     // bumped when launch shapes or kernels change (tuning, ordering)
 
   ~mgmc_handle() {
     for (hipEvent_t e : C.ev) (void)hipEventDestroy(e);
@@ -1592,6 +1609,7 @@ struct mgmc_batch {
   bool planned = false;
   std::vector<double*> obs;
   std::vector<int> fshared;
+  std::vector<uint64_t> gens;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   ~mgmc_batch() {
@@ -1692,7 +1710,11 @@ static void batch_plan(mgmc_batch* b, double* const* obs) {
   if (obs)
     for (size_t q = 0; q < b->hs.size(); ++q) b->obs[q] = obs[q];
   b->fshared.clear();
-  for (mgmc_handle* h : b->hs) b->fshared.push_back(h->E->f_shared() ? 1 : 0);
+  b->gens.clear();
+  for (mgmc_handle* h : b->hs) {
+    b->fshared.push_back(h->E->f_shared() ? 1 : 0);
+    b->gens.push_back(h->generation);
+  }
   b->planned = true;
 }
 
@@ -1759,7 +1781,7 @@ int mgmc_batch_run_dev(mgmc_batch* b, int nsamples, double* const* obs_dev, int 
   bool same = b->planned;
   for (size_t i = 0; same && i < b->hs.size(); ++i) {
     double* o = obs_dev && b->hs[i]->C.nprobe ? obs_dev[i] : nullptr;
-    same = b->obs[i] == o && b->fshared[i] == (b->hs[i]->E->f_shared() ? 1 : 0);
+    same = b->obs[i] == o && b->fshared[i] == (b->hs[i]->E->f_shared() ? 1 : 0) && b->gens[i] == b->hs[i]->generation;
   }
   if (!same) {
     std::vector<double*> o(b->hs.size(), nullptr);
@@ -2177,6 +2199,7 @@ int mgmc_set_ordering(mgmc_handle* h, int ordering) {
   }
   h->E->set_ordering(ordering);
   h->graph_valid = 0;
+  ++h->generation;
   API_END
 }
 
@@ -2210,6 +2233,13 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   } else if (k == "rb_blocks") {
     if (value < 1) throw Error(MGMC_ERR_ARG, "rb_blocks must be >= 1");
     h->C.tune_rb_blocks = value;
+  } else if (k == "lr_dot_chains") {
+    if (value != 1 && value != 2 && value != 4) throw Error(MGMC_ERR_ARG, "lr_dot_chains must be 1, 2 or 4");
+    h->C.tune_lr_dot_chains = value;
+  } else if (k == "lr_update_chains") {
+    if (value != 2 && value != 4 && value != 8 && value != 16)
+      throw Error(MGMC_ERR_ARG, "lr_update_chains must be 2, 4, 8 or 16");
+    h->C.tune_lr_update_chains = value;
   } else if (k == "rb_chains") {
     if (value != 1 && value != 2) throw Error(MGMC_ERR_ARG, "rb_chains must be 1 or 2");
     h->C.tune_rb_chains = value;
@@ -2217,6 +2247,7 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
   h->graph_valid = 0;
+  ++h->generation;
 }
 
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value) {
@@ -2230,6 +2261,8 @@ static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_threads") return h->C.tune_rb_threads;
   if (k == "rb_blocks") return h->C.tune_rb_blocks;
   if (k == "rb_chains") return h->C.tune_rb_chains;
+  if (k == "lr_dot_chains") return h->C.tune_lr_dot_chains;
+  if (k == "lr_update_chains") return h->C.tune_lr_update_chains;
   if (k == "rr_threads") return h->C.tune_rr_threads;
   if (k == "plane_sweep_stage3") return h->C.tune_planes_stage3;
   throw Error(MGMC_ERR_ARG, "no autotuned key '" + k + "'");

@@ -86,3 +86,29 @@ def test_batch_forms_agree():
         for k in range(len(inst)):
             assert np.array_equal(ref[k], z[k]), (mode, persistent, graph, k)
         b.close()
+
+
+def test_batch_follows_tuning_and_rejects_unbatchable_handles():
+    """A member's launch shapes changed after the batch was made (a tuning
+    key) are picked up by the next run, with unchanged samples; a handle in
+    the lexicographic order (wavefront kernels that cannot be merged) is
+    refused with MGMC_ERR_UNSUPPORTED."""
+    import ctypes as C
+
+    from multigridmc_amd import _lib
+
+    inst = _instances(seed=5, count=3, nchains=2, sizes=(32, 48))
+    ref = [_sequential(i, 2, 10) for i in inst]
+    b = SamplerBatch(inst)
+    b.samplers[0].set_tuning("rb_threads", 256)
+    b.samplers[2].set_tuning("rb_blocks", 64)
+    b.run(2, record=False)
+    z = b.run(10)
+    for k in range(len(inst)):
+        assert np.array_equal(ref[k], z[k]), k
+    b.close()
+    i = inst[0]
+    s = mg.MultigridMCSampler(i["op"], i["params"], nchains=1, precision="fp32", seed=1, ordering="lexicographic")
+    hs = (C.c_void_p * 1)(s.h.value)
+    out = C.c_void_p()
+    assert _lib.lib().mgmc_batch_create(1, hs, C.byref(out)) == 2
