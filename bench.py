@@ -228,20 +228,31 @@ def main():
         n = args.calibrate_copy * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        # the same copy timed with HIP events: the streaming read + write rate
-        # this device reaches outside the sampler
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(20):
-            b.copy_(a)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 20
+        c = torch.empty_like(a)
+
+        def rate(op, nbytes):
+            for _ in range(3):
+                op()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                op()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 20
+            return ms, nbytes / (ms * 1e-3) / 1e9
+
+        # the streaming rates this device reaches outside the sampler, timed
+        # with HIP events: read + write (copy), two reads + one write (the
+        # level-0 sweep's mix: x and f read, x written), read only
+        ms, gbps = rate(lambda: b.copy_(a), 8 * n)
+        ms2, gbps2 = rate(lambda: torch.add(a, b, alpha=2.0, out=c), 12 * n)
+        ms3, gbps3 = rate(lambda: torch.sum(a), 4 * n)
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copy_ms": ms,
-                          "copy_GBps_read_plus_write": 8 * n / (ms * 1e-3) / 1e9}))
+                          "copy_GBps_read_plus_write": gbps,
+                          "two_reads_one_write_ms": ms2, "two_reads_one_write_GBps": gbps2,
+                          "read_only_sum_ms": ms3, "read_only_GBps": gbps3}))
         return
 
     if args.mixed_batch:
