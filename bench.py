@@ -48,6 +48,15 @@ def tau_max(obs):
     return float(max(taus)), float(np.median(taus))
 
 
+def tau_max_window(obs, window=10):
+    """Largest fixed-window estimate over the probes (the reference's
+    Statistics::tau_int, src/auxilliary/statistics.cc:65-79), each probe's
+    value the mean over its chains."""
+    import multigridmc_amd as mg
+    return float(max(np.mean([mg.tau_int_window(obs[:, c, p], window) for c in range(obs.shape[1])])
+                     for p in range(obs.shape[2])))
+
+
 def covariance_error(obs, mean_exact, cov_exact, tau):
     """Error of the probe statistics of obs[sample, chain, probe] against the
     exact posterior: relative Frobenius error of the probe covariance, its
@@ -357,7 +366,9 @@ def main():
                     "fp32_batched_max_long_run": tau32_long, "fp32_long_run_samples_per_probe": K * args.stat_steps,
                     "fp64_single_max": tau64, "fp64_single_median": tau64_med,
                     "fp64_samples_per_probe": args.fp64_steps,
-                    "estimator": "Sokal automatic window c=5, chain-averaged autocovariance about the pooled mean"},
+                    "estimator": "Sokal automatic window c=5, chain-averaged autocovariance about the pooled mean",
+                    "reference_window10_fp32_max": tau_max_window(obs32[:args.tau_steps]),
+                    "reference_window10_fp64_max": tau_max_window(obs64)},
         "fp64_single_chain": {"value": ess64, "unit": "indep. samples/s", "steps": args.fp64_steps,
                               "ms_per_step": 1e3 * t64 / args.fp64_steps},
         "speedup_vs_fp64_single": ess32 / ess64,
