@@ -299,15 +299,18 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
  * 64..1024), "plane_sweep" (0/1: sweep 3D box levels in two plane-parity
  * launches), "plane_sweep_min" (fewest plane x chain workgroups for that),
  * "plane_sweep_stage3" (0/1: planes k-1, k, k+1 in LDS), "coarse_mfma" (0/1:
- * coarsest-level products on the FP64 matrix cores), "lr_dot_chains" (1, 2, 4:
+ * coarsest-level products on the FP64 matrix cores), "rb_min_planes" /
+ * "rr_min_planes" (fewest planes per z-segment of the fused level sweep /
+ * residual-restriction), "lr_dot_chains" (1, 2, 4:
  * chains per workgroup of the low-rank dot B^T x), "lr_update_chains" (2, 4,
  * 8, 16: chains per thread of the low-rank update, with more than 8 chains).
  * Results do not depend on these settings beyond rounding. */
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
 /* Autotuner (no reference counterpart): coordinate search over the tuning
- * keys "rb_threads", "rb_blocks", "rr_threads" and "plane_sweep_stage3",
- * which change launch shapes but no computed value; every candidate is timed
+ * keys "rb_chains", "rb_threads", "rb_blocks", "rb_min_planes", "rr_threads",
+ * "rr_min_planes" and "plane_sweep_stage3", which change launch shapes but no
+ * computed value; every candidate is timed
  * over `ncycles` sampling cycles on the handle's stream and the fastest
  * setting is kept. The chains' state and the sample counter are restored,
  * so the sample path after tuning equals the untuned one bit for bit.

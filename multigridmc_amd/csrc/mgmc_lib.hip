@@ -205,6 +205,11 @@ struct Common {
   int tune_fused_rr = 1;      // fused residual + restriction on 3D star levels
   int tune_coarse_mfma = 1;   // coarsest-level products on the FP64 matrix cores
   int tune_rr_threads = 512;
+  // fewest owned planes per z-segment of k_sweep_rb3 and (coarse planes) of
+  // k_residual_restrict_star3: short segments fill the GPU with few chains
+  // (one FP64 chain at 128^3: 0.587 -> 0.549 ms per cycle; no change at 64)
+  int tune_rb_min_planes = 2;
+  int tune_rr_min_planes = 2;
   int tune_lr_dot_chains = 1;     // chains per workgroup of k_lr_dot (1, 2, 4)
   int tune_lr_update_chains = 8;  // chains per thread of k_lr_update (2, 4, 8, 16)
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
@@ -380,7 +385,7 @@ struct Engine : EngineBase {
     int nplanes = g.n2 - 1;
     const int ngroups = C.K / s.CB;
     int want = (C.tune_rb_blocks + s.ntile * ngroups - 1) / (s.ntile * ngroups);
-    int nseg = std::max(1, std::min(want, std::max(1, nplanes / 8)));
+    int nseg = std::max(1, std::min(want, std::max(1, nplanes / C.tune_rb_min_planes)));
     s.SZ = (nplanes + nseg - 1) / nseg;
     s.nseg = (nplanes + s.SZ - 1) / s.SZ;
     s.threads = ((ngx * (s.TY + 4) + 63) / 64) * 64;
@@ -881,7 +886,7 @@ struct Engine : EngineBase {
     const int ntile = (gc.n1 - 1 + TYc - 1) / TYc;
     const int ncpl = gc.n2 - 1;
     int want = (C.tune_rb_blocks + ntile * C.K - 1) / (ntile * C.K);
-    int nseg = std::max(1, std::min(want, std::max(1, ncpl / 4)));
+    int nseg = std::max(1, std::min(want, std::max(1, ncpl / C.tune_rr_min_planes)));
     const int SZc = (ncpl + nseg - 1) / nseg;
     nseg = (ncpl + SZc - 1) / SZc;
     const int ngx = g.px >> 3;
@@ -2233,6 +2238,12 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   } else if (k == "rb_blocks") {
     if (value < 1) throw Error(MGMC_ERR_ARG, "rb_blocks must be >= 1");
     h->C.tune_rb_blocks = value;
+  } else if (k == "rb_min_planes") {
+    if (value < 1) throw Error(MGMC_ERR_ARG, "rb_min_planes must be >= 1");
+    h->C.tune_rb_min_planes = value;
+  } else if (k == "rr_min_planes") {
+    if (value < 1) throw Error(MGMC_ERR_ARG, "rr_min_planes must be >= 1");
+    h->C.tune_rr_min_planes = value;
   } else if (k == "lr_dot_chains") {
     if (value != 1 && value != 2 && value != 4) throw Error(MGMC_ERR_ARG, "lr_dot_chains must be 1, 2 or 4");
     h->C.tune_lr_dot_chains = value;
@@ -2262,6 +2273,8 @@ static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_blocks") return h->C.tune_rb_blocks;
   if (k == "rb_chains") return h->C.tune_rb_chains;
   if (k == "lr_dot_chains") return h->C.tune_lr_dot_chains;
+  if (k == "rb_min_planes") return h->C.tune_rb_min_planes;
+  if (k == "rr_min_planes") return h->C.tune_rr_min_planes;
   if (k == "lr_update_chains") return h->C.tune_lr_update_chains;
   if (k == "rr_threads") return h->C.tune_rr_threads;
   if (k == "plane_sweep_stage3") return h->C.tune_planes_stage3;
@@ -2319,7 +2332,9 @@ int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms) {
   std::vector<Key> keys = {{"rb_chains", {1, 2}},
                            {"rb_threads", {256, 384, 512, 768, 1024}},
                            {"rb_blocks", {384, 512, 768, 1024, 1536, 2048}},
+                           {"rb_min_planes", {1, 2, 4, 8}},
                            {"rr_threads", {256, 512, 768, 1024}},
+                           {"rr_min_planes", {1, 2, 4}},
                            {"plane_sweep_stage3", {1, 0}}};
   double best = time_cycles();
   try {
