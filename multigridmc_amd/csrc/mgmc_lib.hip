@@ -252,6 +252,17 @@ struct EngineBase {
   virtual bool f_shared() const = 0;
   virtual void accumulate_moments(double* m1, double* m2) = 0;
   virtual void unpack_moment(const double* m, double* ref_dev, double scale) = 0;
+  // host-side cycle state (which buffer holds x, lazily zeroed levels,
+  // pending prolongations, the noise tag): saved before a cycle is recorded
+  // for a batch and put back afterwards, so that recording - complete or cut
+  // short by an error - leaves the engine as it found it
+  struct HostState {
+    std::vector<int> cur, xzero, pending;
+    std::vector<double> pending_alpha;
+    uint32_t tag = 0;
+  };
+  virtual HostState save_state() const = 0;
+  virtual void restore_state(const HostState& s) = 0;
 };
 
 // A level field for all chains with a zeroed guard of GUARD elements on both
@@ -1086,6 +1097,15 @@ struct Engine : EngineBase {
     }
   }
 
+  HostState save_state() const override { return HostState{cur, xzero, pending, pending_alpha, C.tag}; }
+  void restore_state(const HostState& s) override {
+    cur = s.cur;
+    xzero = s.xzero;
+    pending = s.pending;
+    pending_alpha = s.pending_alpha;
+    C.tag = s.tag;
+  }
+
   void cycle_sample() override {
     C.tag = 1;
     sample(0);
@@ -1636,14 +1656,17 @@ static void batch_plan(mgmc_batch* b, double* const* obs) {
   for (size_t i = 0; i < b->hs.size(); ++i) {
     Recorder R;
     R.instance = (int)i;
+    const EngineBase::HostState saved = b->hs[i]->E->save_state();
     g_rec = &R;
     try {
       record_cycle(b->hs[i], obs ? obs[i] : nullptr);
     } catch (...) {
       g_rec = nullptr;
+      b->hs[i]->E->restore_state(saved);
       throw;
     }
     g_rec = nullptr;
+    b->hs[i]->E->restore_state(saved);
     for (LaunchRec& r : R.recs) all.push_back(std::move(r));
   }
   b->nrec = (int64_t)all.size();

@@ -218,6 +218,39 @@ void batch_case() {
     check(mgmc_destroy(own[i]), "mgmc_destroy");
     check(mgmc_destroy(bat[i]), "mgmc_destroy");
   }
+  // a handle whose level-1 sweep cannot be merged (the single-pass box march,
+  // reached after level 0 has been recorded): the batch is refused and the
+  // handle's own cycles are those of an identical handle never offered to a batch
+  {
+    ProblemData e;
+    build(cs[0], e);
+    mgmc_handle *hx = nullptr, *hy = nullptr;
+    check(mgmc_create(&e.p, &e.par, &hx), "mgmc_create");
+    check(mgmc_create(&e.p, &e.par, &hy), "mgmc_create");
+    const int64_t N = e.N, probes[2] = {2, N / 2};
+    std::vector<double> f((size_t)N), x((size_t)(N * cs[0].K), 0.0);
+    for (int64_t v = 0; v < N; ++v) f[(size_t)v] = std::sin(0.13 * (double)v);
+    for (mgmc_handle* h : {hx, hy}) {
+      check(mgmc_set_tuning(h, "box_march", 1), "mgmc_set_tuning");
+      check(mgmc_set_rhs(h, f.data(), 0), "mgmc_set_rhs");
+      check(mgmc_set_state(h, x.data()), "mgmc_set_state");
+      check(mgmc_set_probes(h, 2, probes), "mgmc_set_probes");
+    }
+    mgmc_batch* bx = nullptr;
+    if (mgmc_batch_create(1, &hx, &bx) != MGMC_ERR_UNSUPPORTED) {
+      std::printf("FAIL batch: a handle with an unmergeable kernel was accepted\n");
+      ++failures;
+    }
+    std::vector<double> ox((size_t)(2 * cs[0].K * 2)), oy((size_t)(2 * cs[0].K * 2));
+    check(mgmc_run_dev(hx, 2, ox.data(), 0), "mgmc_run_dev");
+    check(mgmc_run_dev(hy, 2, oy.data(), 0), "mgmc_run_dev");
+    if (std::memcmp(ox.data(), oy.data(), ox.size() * sizeof(double)) != 0) {
+      std::printf("FAIL batch: a refused handle's cycles changed\n");
+      ++failures;
+    }
+    check(mgmc_destroy(hx), "mgmc_destroy");
+    check(mgmc_destroy(hy), "mgmc_destroy");
+  }
 }
 
 }  // namespace

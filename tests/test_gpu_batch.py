@@ -112,3 +112,17 @@ def test_batch_follows_tuning_and_rejects_unbatchable_handles():
     hs = (C.c_void_p * 1)(s.h.value)
     out = C.c_void_p()
     assert _lib.lib().mgmc_batch_create(1, hs, C.byref(out)) == 2
+    # refused after level 0 was recorded (the single-pass box march on the
+    # coarser levels): the handle's own cycles stay those of a fresh handle
+    runs = []
+    for offer in (True, False):
+        t = mg.MultigridMCSampler(i["op"], i["params"], nchains=2, precision="fp32", seed=3)
+        t.set_tuning("box_march", 1)
+        t.fix_rhs(i["f"])
+        t.set_state(np.zeros((2, i["op"].lattice.Nvertex)))
+        t.set_observables(i["observables"])
+        if offer:
+            hs = (C.c_void_p * 1)(t.h.value)
+            assert _lib.lib().mgmc_batch_create(1, hs, C.byref(out)) == 2
+        runs.append(t.run(4))
+    assert np.array_equal(runs[0], runs[1])
