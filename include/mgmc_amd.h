@@ -221,6 +221,28 @@ int mgmc_sweep(mgmc_handle* h, int level, int forward, int stochastic, const dou
 int mgmc_residual_restrict(mgmc_handle* h, int level, const double* f, const double* x, double* fc);
 /* x += alpha P xc (src/intergrid/intergrid_operator.hh:106-120). */
 int mgmc_prolongate_add(mgmc_handle* h, int level, double alpha, const double* xc, double* x);
+
+/* ---- level components with the reference's semantics ----
+ * Vectors are nchains x N_level (reference order, FP64); the sweeps run in
+ * the handle's precision and ordering.
+ * mgmc_smoother_apply: Smoother::apply(b, x) (src/smoother/smoother.hh:29) of
+ * SORSmoother (smoother = MGMC_SOR, direction `forward`;
+ * src/smoother/sor_smoother.cc:41-77: nsmooth times nsmooth sparse sweeps,
+ * each followed by the SMW correction) or SSORSmoother (MGMC_SSOR;
+ * src/smoother/ssor_smoother.cc:9-15: nsmooth times a forward and a backward
+ * SORSmoother with nsmooth = 1). */
+int mgmc_smoother_apply(mgmc_handle* h, int level, int smoother, int forward, int nsmooth, const double* b,
+                        double* x);
+/* Sampler::apply(f, x) of SORSampler / SSORSampler on `level`
+ * (src/sampler/sor_sampler.cc:37-59, src/sampler/ssor_sampler.cc:9-15):
+ * nsmooth Gibbs sweeps (SSOR: forward and backward pairs) with fresh Philox
+ * noise; every call advances the handle's sample counter, so consecutive
+ * calls draw independent noise. */
+int mgmc_level_sample(mgmc_handle* h, int level, int smoother, int forward, int nsmooth, const double* f,
+                      double* x);
+/* IntergridOperator::restrict (src/intergrid/intergrid_operator.hh:74-88):
+ * xc = P^T x from `level` to level + 1. */
+int mgmc_restrict(mgmc_handle* h, int level, const double* x, double* xc);
 /* Coarsest level: x = L^{-T}(xi + L^{-1} f) (src/sampler/cholesky_sampler.hh:52-66)
  * with xi injected (nchains x Nc) or from Philox; xi_mode = 0: deterministic
  * x = Q_c^{-1} f (src/solver/cholesky_solver.cc:30-40), 1: Philox, 2: injected. */

@@ -2,9 +2,10 @@
 // (the calls src/driver_mg.cc and src/measurements/sampling_time.cc make on
 // the reference's classes): a 2D 32 x 32 posterior with three ball
 // measurements, solved by the loop solver with the multigrid preconditioner
-// and sampled by MGMC chains through Sampler::apply, fix_rhs and deep_copy.
-// The vectors go to OUTDIR as raw FP64 files for checking; the key of each
-// sampler is printed.
+// and sampled by MGMC chains through Sampler::apply, fix_rhs and deep_copy;
+// then the level components: SOR / SSOR smoothers, SOR / SSOR samplers and
+// the intergrid operator. The vectors go to OUTDIR as raw FP64 files for
+// checking; the key of each sampler is printed.
 #include <cstdio>
 #include <random>
 #include <string>
@@ -13,6 +14,10 @@
 #include "mgmc_adapters.hh"
 
 using namespace mgmc_amd;
+
+static void check_status(int status) {
+  if (status != MGMC_OK) throw Error(status, mgmc_last_error());
+}
 
 static void save(const std::string& dir, const char* name, const std::vector<double>& v) {
   const std::string path = dir + "/" + name;
@@ -102,6 +107,53 @@ int main(int argc, char* argv[]) {
     } catch (const Error& e) {
       std::printf("length error status %d\n", e.status);
     }
+    // Smoother, SOR / SSOR samplers and the intergrid operator (omega 0.8)
+    std::vector<double> x0(N);
+    for (auto& v : x0) v = normal(gen);
+    save(out, "x0.bin", x0);
+    std::vector<double> xs = x0;
+    HipSORSmoother(op, 0.8, 1, Direction::forward).apply(b, xs);
+    save(out, "sor_fwd.bin", xs);
+    xs = x0;
+    HipSORSmoother(op, 0.8, 1, Direction::backward).apply(b, xs);
+    save(out, "sor_bwd.bin", xs);
+    xs = x0;
+    HipSSORSmoother(op, 0.8, 2).apply(b, xs);
+    save(out, "ssor2.bin", xs);
+    // the reference's nsmooth = 2 on the prior: twice (two sparse sweeps, then the correction)
+    xs = x0;
+    HipSORSmoother(prior, 0.8, 2, Direction::forward).apply(b, xs);
+    save(out, "sor2_prior.bin", xs);
+    // SORSampler on the prior: the normals of its first two calls and their results
+    HipSORSampler sors(prior, std::make_shared<Mt19937Generator>(11), 0.8, 1, Direction::forward);
+    std::vector<double> xi(N), xi2(N);
+    check_status(mgmc_normals(sors.handle(), 1, xi.data()));
+    xs = x0;
+    sors.apply(b, xs);
+    save(out, "sor_sampler_x1.bin", xs);
+    check_status(mgmc_normals(sors.handle(), 1, xi2.data()));
+    sors.apply(b, xs);
+    save(out, "sor_sampler_x2.bin", xs);
+    save(out, "sor_sampler_xi1.bin", xi);
+    save(out, "sor_sampler_xi2.bin", xi2);
+    // SSORSampler on the posterior: a path of three samples
+    HipSSORSampler ssors(op, std::make_shared<Mt19937Generator>(12), 1.0, 1);
+    std::vector<double> ssor_path;
+    std::fill(chain.begin(), chain.end(), 0.0);
+    for (int s = 0; s < 3; ++s) {
+      ssors.apply(b, chain);
+      ssor_path.insert(ssor_path.end(), chain.begin(), chain.end());
+    }
+    save(out, "ssor_sampler.bin", ssor_path);
+    std::printf("key_ssor %llu\n", (unsigned long long)ssors.seed());
+    // IntergridOperator: x_c = P^T x0, then x0 + 0.7 P x_c
+    HipIntergridOperator P(op);
+    std::vector<double> xc((size_t)P.coarse_ndof());
+    P.restrict(x0, xc);
+    save(out, "restrict.bin", xc);
+    xs = x0;
+    P.prolongate_add(0.7, xc, xs);
+    save(out, "prolongate.bin", xs);
   } catch (const Error& e) {
     std::printf("ERROR: %s\n", e.what());
     return 1;

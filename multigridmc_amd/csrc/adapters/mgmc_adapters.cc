@@ -257,4 +257,100 @@ void HipLoopSolver::apply(ConstVec b, Vec x) {
   }
 }
 
+// a one-level handle for the smoothers and samplers (the level is swept, no
+// coarse factorisation), a two-level one for the intergrid operator
+static MultigridParameters level_params(int nlevel, double omega) {
+  MultigridParameters p;
+  p.nlevel = nlevel;
+  p.omega = omega;
+  p.npresmooth = p.npostsmooth = 0;
+  p.coarse_solver = "SSOR";
+  return p;
+}
+
+static void check_n(size_t a, size_t b, const char* what) {
+  if (a != b) throw Error(MGMC_ERR_ARG, std::string(what) + ": vectors must hold one value per vertex");
+}
+
+HipSORSmoother::HipSORSmoother(std::shared_ptr<LinearOperator> linear_operator_, double omega, int nsmooth_,
+                               Direction direction_, const DeviceOptions& o)
+    : Smoother(std::move(linear_operator_)), nsmooth(nsmooth_), direction(direction_) {
+  h = std::make_unique<Handle>(*linear_operator, level_params(1, omega), MGMC_COARSE_SSOR, o, 0);
+}
+
+void HipSORSmoother::apply(ConstVec b, Vec x) const {
+  const size_t N = (size_t)linear_operator->get_lattice().Nvertex();
+  check_n(b.n, N, "Smoother::apply");
+  check_n(x.n, N, "Smoother::apply");
+  check(mgmc_smoother_apply(h->get(), 0, MGMC_SOR, direction == Direction::forward, nsmooth, b.p, x.p),
+        "mgmc_smoother_apply");
+}
+
+HipSSORSmoother::HipSSORSmoother(std::shared_ptr<LinearOperator> linear_operator_, double omega, int nsmooth_,
+                                 const DeviceOptions& o)
+    : Smoother(std::move(linear_operator_)), nsmooth(nsmooth_) {
+  h = std::make_unique<Handle>(*linear_operator, level_params(1, omega), MGMC_COARSE_SSOR, o, 0);
+}
+
+void HipSSORSmoother::apply(ConstVec b, Vec x) const {
+  const size_t N = (size_t)linear_operator->get_lattice().Nvertex();
+  check_n(b.n, N, "Smoother::apply");
+  check_n(x.n, N, "Smoother::apply");
+  check(mgmc_smoother_apply(h->get(), 0, MGMC_SSOR, 1, nsmooth, b.p, x.p), "mgmc_smoother_apply");
+}
+
+HipSORSampler::HipSORSampler(std::shared_ptr<LinearOperator> linear_operator_, std::shared_ptr<RandomGenerator> rng_,
+                             double omega_, int nsmooth_, Direction direction_, const DeviceOptions& o)
+    : HipSORSampler(std::move(linear_operator_), std::move(rng_), omega_, nsmooth_, direction_, o, false) {}
+
+HipSORSampler::HipSORSampler(std::shared_ptr<LinearOperator> linear_operator_, std::shared_ptr<RandomGenerator> rng_,
+                             double omega_, int nsmooth_, Direction direction_, const DeviceOptions& o,
+                             bool symmetric_)
+    : Sampler(std::move(linear_operator_), std::move(rng_)), omega(omega_), nsmooth(nsmooth_),
+      direction(direction_), options(o), symmetric(symmetric_) {
+  if (!rng) throw Error(MGMC_ERR_ARG, "the sampler needs a random generator");
+  key = draw_key(*rng);
+  h = std::make_unique<Handle>(*linear_operator, level_params(1, omega), MGMC_COARSE_SSOR, options, key);
+}
+
+std::shared_ptr<Sampler> HipSORSampler::deep_copy(std::shared_ptr<RandomGenerator> rng_) {
+  return std::make_shared<HipSORSampler>(linear_operator, std::move(rng_), omega, nsmooth, direction, options);
+}
+
+void HipSORSampler::apply(ConstVec f, Vec x) const {
+  const size_t N = (size_t)linear_operator->get_lattice().Nvertex();
+  check_n(f.n, N, "Sampler::apply");
+  check_n(x.n, N, "Sampler::apply");
+  check(mgmc_level_sample(h->get(), 0, symmetric ? MGMC_SSOR : MGMC_SOR, direction == Direction::forward, nsmooth,
+                          f.p, x.p),
+        "mgmc_level_sample");
+}
+
+HipSSORSampler::HipSSORSampler(std::shared_ptr<LinearOperator> linear_operator_,
+                               std::shared_ptr<RandomGenerator> rng_, double omega_, int nsmooth_,
+                               const DeviceOptions& o)
+    : HipSORSampler(std::move(linear_operator_), std::move(rng_), omega_, nsmooth_, Direction::forward, o, true) {}
+
+std::shared_ptr<Sampler> HipSSORSampler::deep_copy(std::shared_ptr<RandomGenerator> rng_) {
+  return std::make_shared<HipSSORSampler>(linear_operator, std::move(rng_), omega, nsmooth, options);
+}
+
+HipIntergridOperator::HipIntergridOperator(std::shared_ptr<LinearOperator> fine_operator, const DeviceOptions& o) {
+  h = std::make_unique<Handle>(*fine_operator, level_params(2, 1.0), MGMC_COARSE_SSOR, o, 0);
+  check(mgmc_ndof(h->get(), 0, &n), "mgmc_ndof");
+  check(mgmc_ndof(h->get(), 1, &nc), "mgmc_ndof");
+}
+
+void HipIntergridOperator::restrict(ConstVec x, Vec x_coarse) const {
+  check_n(x.n, (size_t)n, "IntergridOperator::restrict");
+  check_n(x_coarse.n, (size_t)nc, "IntergridOperator::restrict");
+  check(mgmc_restrict(h->get(), 0, x.p, x_coarse.p), "mgmc_restrict");
+}
+
+void HipIntergridOperator::prolongate_add(double alpha, ConstVec x_coarse, Vec x) const {
+  check_n(x.n, (size_t)n, "IntergridOperator::prolongate_add");
+  check_n(x_coarse.n, (size_t)nc, "IntergridOperator::prolongate_add");
+  check(mgmc_prolongate_add(h->get(), 0, alpha, x_coarse.p, x.p), "mgmc_prolongate_add");
+}
+
 }  // namespace mgmc_amd

@@ -6,6 +6,13 @@
 //   Preconditioner           src/preconditioner/preconditioner.hh:17-37
 //   HipMultigridPreconditioner  MultigridPreconditioner, src/preconditioner/multigrid_preconditioner.hh
 //   HipLoopSolver            LoopSolver, src/solver/loop_solver.hh
+//   Smoother                 src/smoother/smoother.hh:18-37
+//   HipSORSmoother, HipSSORSmoother  SORSmoother, SSORSmoother
+//                            (src/smoother/sor_smoother.hh, ssor_smoother.hh)
+//   HipSORSampler, HipSSORSampler    SORSampler, SSORSampler
+//                            (src/sampler/sor_sampler.hh, ssor_sampler.hh)
+//   HipIntergridOperator     IntergridOperatorLinear's restrict / prolongate_add
+//                            (src/intergrid/intergrid_operator.hh:74-120)
 //   MultigridParameters      src/auxilliary/parameters.hh:150-179
 //   IterativeSolverParameters src/auxilliary/parameters.hh:110-133
 //   RandomGenerator          src/auxilliary/parallel_random.hh:17-55 (draw_int only)
@@ -241,6 +248,96 @@ class HipLoopSolver {
   IterativeSolverParameters params;
   int niter_ = -1;
   std::vector<double> hist;
+};
+
+// Sweep direction of the SOR smoothers and samplers (src/smoother/sor_smoother.hh).
+enum class Direction { forward, backward };
+
+class Smoother {
+ public:
+  explicit Smoother(std::shared_ptr<LinearOperator> linear_operator_) : linear_operator(std::move(linear_operator_)) {}
+  virtual ~Smoother() = default;
+  // x: the initial guess on entry, the smoothed solution of Q x = b on exit
+  virtual void apply(ConstVec b, Vec x) const = 0;
+
+ protected:
+  std::shared_ptr<LinearOperator> linear_operator;
+};
+
+// SORSmoother::apply (src/smoother/sor_smoother.cc:41-77) on the GPU: nsmooth
+// times (nsmooth sparse sweeps in `direction`, then the SMW correction for
+// the measurements), in the device ordering of DeviceOptions (multicolour
+// unless lexicographic is asked for).
+class HipSORSmoother : public Smoother {
+ public:
+  HipSORSmoother(std::shared_ptr<LinearOperator> linear_operator, double omega, int nsmooth, Direction direction,
+                 const DeviceOptions& options = DeviceOptions());
+  void apply(ConstVec b, Vec x) const override;
+
+ private:
+  int nsmooth;
+  Direction direction;
+  std::unique_ptr<Handle> h;
+};
+
+// SSORSmoother::apply (src/smoother/ssor_smoother.cc:9-15): nsmooth times a
+// forward and a backward SOR sweep with their SMW corrections.
+class HipSSORSmoother : public Smoother {
+ public:
+  HipSSORSmoother(std::shared_ptr<LinearOperator> linear_operator, double omega, int nsmooth,
+                  const DeviceOptions& options = DeviceOptions());
+  void apply(ConstVec b, Vec x) const override;
+
+ private:
+  int nsmooth;
+  std::unique_ptr<Handle> h;
+};
+
+// SORSampler / SSORSampler::apply (src/sampler/sor_sampler.cc:37-59,
+// ssor_sampler.cc:9-15): Gibbs sweeps of N(Q^{-1} f, Q^{-1}) with fresh
+// Philox noise on every call, keyed by two draws of `rng`.
+class HipSORSampler : public Sampler {
+ public:
+  HipSORSampler(std::shared_ptr<LinearOperator> linear_operator, std::shared_ptr<RandomGenerator> rng, double omega,
+                int nsmooth, Direction direction, const DeviceOptions& options = DeviceOptions());
+  std::shared_ptr<Sampler> deep_copy(std::shared_ptr<RandomGenerator> rng) override;
+  void apply(ConstVec f, Vec x) const override;
+  uint64_t seed() const { return key; }
+  mgmc_handle* handle() const { return h->get(); }
+
+ protected:
+  HipSORSampler(std::shared_ptr<LinearOperator> linear_operator, std::shared_ptr<RandomGenerator> rng, double omega,
+                int nsmooth, Direction direction, const DeviceOptions& options, bool symmetric);
+  double omega;
+  int nsmooth;
+  Direction direction;
+  DeviceOptions options;
+  bool symmetric;
+  uint64_t key;
+  std::unique_ptr<Handle> h;
+};
+class HipSSORSampler : public HipSORSampler {
+ public:
+  HipSSORSampler(std::shared_ptr<LinearOperator> linear_operator, std::shared_ptr<RandomGenerator> rng, double omega,
+                 int nsmooth, const DeviceOptions& options = DeviceOptions());
+  std::shared_ptr<Sampler> deep_copy(std::shared_ptr<RandomGenerator> rng) override;
+};
+
+// The intergrid operator of a fine lattice and its coarsening
+// (IntergridOperatorLinear, src/intergrid/intergrid_operator_linear.cc:8-31):
+// restrict(x, x_coarse) sets x_coarse = P^T x, prolongate_add(alpha,
+// x_coarse, x) adds alpha P x_coarse to x (intergrid_operator.hh:74-120).
+class HipIntergridOperator {
+ public:
+  explicit HipIntergridOperator(std::shared_ptr<LinearOperator> fine_operator,
+                                const DeviceOptions& options = DeviceOptions());
+  void restrict(ConstVec x, Vec x_coarse) const;
+  void prolongate_add(double alpha, ConstVec x_coarse, Vec x) const;
+  int64_t coarse_ndof() const { return nc; }
+
+ private:
+  int64_t n = 0, nc = 0;
+  std::unique_ptr<Handle> h;
 };
 
 }  // namespace mgmc_amd

@@ -238,6 +238,10 @@ struct EngineBase {
   virtual void probe(double* obs_dev) = 0;
   virtual void* field(int level, int which) = 0;  // 0 x, 1 f, 2 r
   virtual void sweep(int level, bool forward, bool stochastic, const double* xi_dev, const double* xilr_dev) = 0;
+  // SORSmoother / SSORSmoother::apply and SORSampler / SSORSampler::apply on one level
+  virtual void smoother_apply(int level, int smoother, bool forward, int nsmooth, bool stochastic) = 0;
+  // x_{level+1} = P^T x_level on the f fields (IntergridOperator::restrict)
+  virtual void restrict_only(int level) = 0;
   virtual void residual_restrict(int level) = 0;
   virtual void prolongate(int level, double alpha) = 0;
   virtual void coarse(int mode, const double* xi_dev) = 0;
@@ -1172,6 +1176,33 @@ struct Engine : EngineBase {
     if (noise == NOISE_PHILOX && xilr_dev) throw Error(MGMC_ERR_ARG, "xi_lr needs xi");
     dir_sweep(l, forward, noise, extp, xilr_dev);
     normalise();
+  }
+
+  // SORSmoother::apply (src/smoother/sor_smoother.cc:41-53): nsmooth times
+  // (nsmooth sparse sweeps, then the SMW correction); SSORSmoother::apply
+  // (ssor_smoother.cc:9-15): nsmooth times (forward, backward), each an
+  // SORSmoother with nsmooth = 1 (ssor_smoother.hh:44-49). With `stochastic`
+  // the samplers SORSampler / SSORSampler::apply (sor_sampler.cc:37-59,
+  // ssor_sampler.cc:9-15): every directional sweep draws fresh noise and is
+  // followed by its SMW step. b (or f) and x are the level's f and x fields.
+  void smoother_apply(int l, int smoother, bool forward, int nsmooth, bool stochastic) override {
+    const int noise = stochastic ? NOISE_PHILOX : NOISE_NONE;
+    for (int s = 0; s < nsmooth; ++s) {
+      if (smoother == MGMC_SSOR) {
+        dir_sweep(l, true, noise, nullptr, nullptr);
+        dir_sweep(l, false, noise, nullptr, nullptr);
+      } else {
+        dir_sweep(l, forward, noise, nullptr, nullptr, stochastic ? 1 : nsmooth);
+      }
+    }
+    normalise();
+  }
+  void restrict_only(int l) override {
+    Level& F = *C.L[l];
+    Level& Cc = *C.L[l + 1];
+    dim3 gq = row_grid(Cc.g, (Cc.g.n0 + 3) / 4);
+    if (F.g.dim == 3) launch(gq, 256, 0, C.stream, kb_restrict<T, 3>, f[l].p, f[l + 1].p, F.g, Cc.g);
+    else launch(gq, 256, 0, C.stream, kb_restrict<T, 2>, f[l].p, f[l + 1].p, F.g, Cc.g);
   }
 
   // ---- FP64 fine-level operations for the loop solver (T-independent data) ----
@@ -2177,6 +2208,58 @@ int mgmc_prolongate_add(mgmc_handle* h, int level, double alpha, const double* x
   h->E->prolongate(level, alpha);
   h->E->unpack(level, 0, dx, 1.0);
   HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+  API_END
+}
+
+// SORSmoother / SSORSmoother::apply (smoother = MGMC_SOR / MGMC_SSOR) and,
+// with stochastic, SORSampler / SSORSampler::apply on one level: the noise
+// of a call is keyed by the sample counter, which the call then advances
+static void level_apply(mgmc_handle* h, int level, int smoother, int forward, int nsmooth, int stochastic,
+                        const double* b, double* x) {
+  Common& C = h->C;
+  if (level < 0 || level >= C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  if (smoother != MGMC_SOR && smoother != MGMC_SSOR) throw Error(MGMC_ERR_ARG, "invalid smoother");
+  if (nsmooth < 0) throw Error(MGMC_ERR_ARG, "nsmooth < 0");
+  if (!b || !x) throw Error(MGMC_ERR_ARG, "b and x are needed");
+  const size_t n = (size_t)C.L[level]->N * C.K;
+  double* db = stage(h->io1, b, n, C.stream);
+  h->E->pack(level, 1, db, 1);
+  double* dx = stage(h->io2, x, n, C.stream);
+  h->E->pack(level, 0, dx, 1);
+  C.tag = 1;
+  h->E->smoother_apply(level, smoother, forward != 0, nsmooth, stochastic != 0);
+  if (stochastic) launch(1, 64, 0, C.stream, kb_advance, C.ctr.p);
+  h->E->unpack(level, 0, dx, 1.0);
+  HIP_CHECK(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, C.stream));
+  sync(h);
+}
+
+int mgmc_smoother_apply(mgmc_handle* h, int level, int smoother, int forward, int nsmooth, const double* b,
+                        double* x) {
+  API_BEGIN
+  level_apply(h, level, smoother, forward, nsmooth, 0, b, x);
+  API_END
+}
+
+int mgmc_level_sample(mgmc_handle* h, int level, int smoother, int forward, int nsmooth, const double* f,
+                      double* x) {
+  API_BEGIN
+  level_apply(h, level, smoother, forward, nsmooth, 1, f, x);
+  API_END
+}
+
+int mgmc_restrict(mgmc_handle* h, int level, const double* x, double* xc) {
+  API_BEGIN
+  Common& C = h->C;
+  if (level < 0 || level + 1 >= C.nlevel) throw Error(MGMC_ERR_ARG, "invalid level");
+  size_t n = (size_t)C.L[level]->N * C.K, nc = (size_t)C.L[level + 1]->N * C.K;
+  double* dx = stage(h->io1, x, n, C.stream);
+  h->E->pack(level, 1, dx, 1);
+  h->E->restrict_only(level);
+  double* dc = stage(h->io3, nullptr, nc, C.stream);
+  h->E->unpack(level + 1, 1, dc, 1.0);
+  HIP_CHECK(hipMemcpyAsync(xc, dc, nc * sizeof(double), hipMemcpyDeviceToHost, C.stream));
   sync(h);
   API_END
 }
