@@ -202,6 +202,61 @@ def mixed_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def config3(args):
+    """The mixed-precision configuration at scale: 256^3, -div(a grad) +
+    kappa^2 with a piecewise-constant a in [1, 10] on a seeded 8^3 block grid,
+    64 balls of radius 0.03 (variance 1e-4), 6 levels. (1) The loop solver
+    with FP32 cycles and FP64 residuals to 1e-10 relative, against the same
+    solve with FP64 cycles; (2) 32 FP32 chains x --config3-samples cycles at
+    16 seeded probe vertices, against the exact posterior moments from FP64
+    multigrid solves (tolerance 1e-11). Prints one JSON line."""
+    import multigridmc_amd as mg
+    from multigridmc_amd import measurements as ms
+    from multigridmc_amd.operators import block_diffusion
+    lat = mg.Lattice(256, 256, 256)
+    a = block_diffusion(lat, 8, 1.0, 10.0, 33)
+    centres, y, var = mg.seeded_ball_measurements(lat, 64, 0.03, 1e-4, 31)
+    op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0, diffusion=a), centres, 0.03, var)
+    N = lat.Nvertex
+    p = mg.MultigridParameters(nlevel=6)
+    out = {"workload": "256^3, block diffusion a in [1,10] on 8^3 blocks, 64 balls r=0.03 var=1e-4, 6 levels"}
+    b = np.random.default_rng(3).standard_normal(N)
+    sols = {}
+    for precision in ("fp32", "fp64"):
+        solver = mg.LoopSolver(op, mg.MultigridPreconditioner(op, p, precision=precision), rtol=1e-10, atol=1e300,
+                               maxiter=100)
+        t0 = time.perf_counter()
+        sols[precision] = solver.apply(b)
+        t = time.perf_counter() - t0
+        h = solver.history
+        out[f"solve_{precision}_cycles"] = {"iterations": len(h) - 1, "final_rel_residual": float(h[-1] / h[0]),
+                                            "seconds": t}
+        print(f"config3: solve with {precision} cycles done", file=sys.stderr, flush=True)
+    out["solve_fp32_cycles_vs_fp64_rel_error"] = float(np.linalg.norm(sols["fp32"] - sols["fp64"]) /
+                                                       np.linalg.norm(sols["fp64"]))
+    probes = np.sort(np.random.default_rng(6).choice(N, size=16, replace=False))
+    obsv = [(np.array([q]), np.array([1.0])) for q in probes]
+    mean_exact, cov_exact = ms.observed_moments(op, np.zeros(N), y, obsv, params=p, rtol=1e-11)
+    print("config3: exact probe moments done", file=sys.stderr, flush=True)
+    K, ns = 32, args.config3_samples
+    s = new_sampler(op, op.get_B() @ (y / op.get_Sigma()), p, K, "fp32", probes, 19)
+    s.run(50, record=False)
+    chunks, done, t = [], 0, 0.0
+    while done < ns:  # in parts, with a progress line after each
+        m = min(500, ns - done)
+        t0 = time.perf_counter()
+        chunks.append(s.run(m))
+        t += time.perf_counter() - t0
+        done += m
+        print(f"config3: {done} of {ns} cycles sampled", file=sys.stderr, flush=True)
+    obs = np.concatenate(chunks)
+    tau = max(mg.tau_int_chains(obs[:, :, q], mean=mean_exact[q])[0] for q in range(len(probes)))
+    out["sampling"] = {"chains": K, "samples_per_chain": ns, "probes": len(probes), "ms_per_cycle": 1e3 * t / ns,
+                       "tau_int_max": tau,
+                       "covariance_error_vs_fp64_exact": covariance_error(obs, mean_exact, cov_exact, tau)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +281,10 @@ def main():
     ap.add_argument("--mixed-batch", action="store_true",
                     help="only time the mixed batch of 16 instances (32^3..128^3, 8-32 balls, 4 FP32 chains each): "
                          "merged launches (plain and persistent) against one stream per instance")
+    ap.add_argument("--config3", action="store_true",
+                    help="only run the 256^3 block-diffusion configuration once: the mixed-precision solve and "
+                         "32 FP32 chains x --config3-samples with probe errors against the FP64 exact posterior")
+    ap.add_argument("--config3-samples", type=int, default=5000)
     ap.add_argument("--persistent", type=int, default=1,
                     help="workgroups per CU of the persistent merged launches (--mixed-batch)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -266,6 +325,9 @@ def main():
 
     if args.mixed_batch:
         mixed_batch(args)
+        return
+    if args.config3:
+        config3(args)
         return
 
     if args.gpus != 1:
