@@ -11,8 +11,13 @@ import numpy as np
 
 
 def tau_int_window(q, window):
+    """The window is clamped to the length of the series (a series of at
+    most one sample gives 1)."""
     q = np.asarray(q, dtype=np.float64)
     n = len(q)
+    window = min(int(window), n)
+    if window <= 1:
+        return 1.0
     avg = q.mean()
     S = np.array([np.dot(q[k:], q[: n - k]) / (n - k) for k in range(window)])
     C = S - avg * avg

@@ -102,6 +102,9 @@ def test_stats_estimators_on_ar1():
     # windowed estimator equals the reference's running-average form
     w = 12
     assert abs(mg.tau_int_window(x[:20000], w) - po.tau_int_reference(x[:20000], w)) < 1e-9
+    # a window longer than the series is clamped to its length
+    assert mg.tau_int_window(x[:5], 10) == mg.tau_int_window(x[:5], 5)
+    assert mg.tau_int_window(x[:1], 10) == 1.0
     chains = np.stack([x[i * 40000:(i + 1) * 40000] for i in range(10)], axis=1)
     tau_c, _ = mg.tau_int_chains(chains)
     assert abs(tau_c - tau_exact) < 0.15
