@@ -154,6 +154,10 @@ def cpu_baseline(n, m, seed, budget_s, nthreads):
 
 
 def mixed_batch(args):
+    print(json.dumps(mixed_batch_run(args.steps, args.warmup, not args.no_graph, args.persistent)), flush=True)
+
+
+def mixed_batch_run(steps, warmup, use_graph, persistent, forms=("merged", "merged_persistent", "streams")):
     """The uneven batch of the batched-chains configuration: 16 posterior
     instances on lattices drawn from {32^3, 48^3, 64^3, 96^3, 128^3} (seeded),
     8-32 ball measurements each, 4 FP32 chains per instance, V(1,1) SSOR down
@@ -178,28 +182,31 @@ def mixed_batch(args):
     chains = sum(i["nchains"] for i in inst)
     out = {"workload": "mixed batch of 16 instances, lattices " +
                        ",".join(str(i["op"].lattice.n[0]) for i in inst) + ", 4 FP32 chains each",
-           "chains": chains, "steps": args.steps, "warmup": args.warmup, "graph": not args.no_graph}
-    for name, mode, persistent in (("merged", "merged", 0), ("merged_persistent", "merged", args.persistent),
-                                   ("streams", "streams", 0)):
-        b = SamplerBatch(inst, mode=mode, persistent=persistent)
-        bufs = [torch.empty((max(args.steps, args.warmup), s.nchains, 1), dtype=torch.float64, device="cuda")
+           "chains": chains, "steps": steps, "warmup": warmup, "graph": use_graph}
+    for name, mode, pers in (("merged", "merged", 0), ("merged_persistent", "merged", persistent),
+                             ("streams", "streams", 0)):
+        if name not in forms:
+            continue
+        b = SamplerBatch(inst, mode=mode, persistent=pers)
+        bufs = [torch.empty((max(steps, warmup), s.nchains, 1), dtype=torch.float64, device="cuda")
                 for s in b.samplers]
         torch.cuda.synchronize()
-        b.run_dev(args.warmup, bufs, not args.no_graph)
+        b.run_dev(warmup, bufs, use_graph)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        b.run_dev(args.steps, bufs, not args.no_graph)
+        b.run_dev(steps, bufs, use_graph)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         st = b.stats()
-        out[name] = {"ms_per_cycle": 1e3 * t / args.steps, "chain_samples_per_s": chains * args.steps / t}
+        out[name] = {"ms_per_cycle": 1e3 * t / steps, "chain_samples_per_s": chains * steps / t}
         if st:
             out[name]["launches_per_cycle"] = st[0]
             out[name]["separate_launches_per_cycle"] = st[1]
         b.close()
         del bufs
-    out["speedup_merged_vs_streams"] = out["streams"]["ms_per_cycle"] / out["merged"]["ms_per_cycle"]
-    print(json.dumps(out), flush=True)
+    if "merged" in out and "streams" in out:
+        out["speedup_merged_vs_streams"] = out["streams"]["ms_per_cycle"] / out["merged"]["ms_per_cycle"]
+    return out
 
 
 def config3(args):
@@ -275,6 +282,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-batch timing in the metric line")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--colour-phases", action="store_true",
                     help="sweep level 0 with one launch per colour instead of the single-pass kernel")
@@ -446,6 +454,14 @@ def main():
         out["covariance_error_vs_fp64_exact"] = {
             "fp32_batched": covariance_error(obs32, mean_exact, cov_exact, tau32_long),
             "fp64_single": covariance_error(obs64, mean_exact, cov_exact, tau64)}
+    if not args.no_mixed:
+        # the uneven part of the batched configuration beside the headline:
+        # 16 instances of different sizes as merged launches against one
+        # stream per instance (bench.py --mixed-batch for all forms)
+        try:
+            out["mixed_batch"] = mixed_batch_run(100, 10, g, 0, forms=("merged", "streams"))
+        except Exception as e:  # the metric line must still print
+            out["mixed_batch"] = {"error": str(e)}
     if not args.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         try:

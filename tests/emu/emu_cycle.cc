@@ -237,6 +237,19 @@ void batch_case() {
       check(mgmc_set_probes(h, 2, probes), "mgmc_set_probes");
     }
     mgmc_batch* bx = nullptr;
+    // W-cycles revisit levels in an order that differs between instances
+    {
+      ProblemData w;
+      build(cs[0], w);
+      w.par.cycle = 2;
+      mgmc_handle* hw = nullptr;
+      check(mgmc_create(&w.p, &w.par, &hw), "mgmc_create");
+      if (mgmc_batch_create(1, &hw, &bx) != MGMC_ERR_UNSUPPORTED) {
+        std::printf("FAIL batch: a W-cycle handle was accepted\n");
+        ++failures;
+      }
+      check(mgmc_destroy(hw), "mgmc_destroy");
+    }
     if (mgmc_batch_create(1, &hx, &bx) != MGMC_ERR_UNSUPPORTED) {
       std::printf("FAIL batch: a handle with an unmergeable kernel was accepted\n");
       ++failures;
