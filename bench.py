@@ -339,8 +339,9 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (chains would shard across "
-              f"GPUs with no per-step communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the chains would shard "
+              f"across GPUs, one process per GPU, with no per-step communication; no multi-GPU path is built)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch  # noqa: F401  (before the library: one HIP runtime in the process)
