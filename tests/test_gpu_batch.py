@@ -126,3 +126,26 @@ def test_batch_follows_tuning_and_rejects_unbatchable_handles():
             assert _lib.lib().mgmc_batch_create(1, hs, C.byref(out)) == 2
         runs.append(t.run(4))
     assert np.array_equal(runs[0], runs[1])
+
+
+def test_batch_mixes_2d_and_3d_instances():
+    """2D instances (colour-phase sweeps, 2D transfers, a 49-unknown coarsest
+    level) merged with 3D ones: every instance equals its own run."""
+    inst = _instances(seed=41, count=2, nchains=2, sizes=(32,))
+    for k, n in enumerate((64, 48)):
+        lat = mg.Lattice(n, n)
+        centres, y, var = mg.seeded_ball_measurements(lat, 5, 0.1, 1e-3, 90 + k)
+        op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.1, var)
+        obs = [mg.measurement_vector(lat, [0.4, 0.6], 0.1), mg.measurement_vector(lat, [0.5, 0.5], 0.0)]
+        inst.append(dict(op=op, params=mg.MultigridParameters(nlevel=ms._nlevel(op)), nchains=3 - k,
+                         f=op.get_B() @ (y / var), observables=obs, seed=500 + k,
+                         precision="fp64" if k else "fp32"))
+    ref = [_sequential(i, 2, 12) for i in inst]
+    b = SamplerBatch(inst)
+    merged, separate = b.stats()
+    assert merged < separate
+    b.run(2, record=False)
+    z = b.run(12)
+    for k in range(len(inst)):
+        assert np.array_equal(ref[k], z[k]), k
+    b.close()
