@@ -314,7 +314,9 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
  * sweep 3D star levels with the single-pass red-black kernel), "rb_threads"
  * (threads per workgroup of that kernel, 64..1024; default 512), "rb_blocks"
  * (workgroups it and the fused residual-restriction aim for, default 768;
- * more z-segments when fewer tiles x chains exist),
+ * more z-segments when fewer tiles x chains exist), "rb_balanced" (0/1,
+ * default 1: split the fused sweep's chains x tiles x planes evenly over one
+ * round of resident workgroups instead of z-segments of rb_blocks),
  * "fold_prolongation" (0/1: add the coarse correction inside the first
  * post-smoothing sweep), "fused_residual" (0/1: residual + restriction in one
  * pass on 3D star levels), "rr_threads" (its threads per workgroup,
@@ -330,8 +332,8 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
 /* Autotuner (no reference counterpart): coordinate search over the tuning
- * keys "rb_chains", "rb_threads", "rb_blocks", "rb_min_planes", "rr_threads",
- * "rr_min_planes" and "plane_sweep_stage3", which change launch shapes but no
+ * keys "rb_chains", "rb_balanced", "rb_threads", "rb_blocks", "rb_min_planes",
+ * "rr_threads", "rr_min_planes" and "plane_sweep_stage3", which change launch shapes but no
  * computed value; every candidate is timed
  * over `ncycles` sampling cycles on the handle's stream and the fastest
  * setting is kept. The chains' state and the sample counter are restored,

@@ -1303,7 +1303,7 @@ template <typename T, int NOISE, int CB>
 __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
-                                                    NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
+                                                    int ncol, NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
                                                     T alpha) {
   static_assert(CB == 1 || CB == 2, "one or two chains per workgroup");
   MGMC_DYNAMIC_SHARED(char, lds_raw);
@@ -1323,25 +1323,27 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int r = tid / ngx, gx = tid % ngx;
-  int b = bk.x;
-  const int tile = b % ntile;
-  b /= ntile;
-  const int seg = b % nseg;
-  const int chain = (b / nseg) * CB;  // first chain of the workgroup
+  const bool active = r < TY + 4;
+  const long long pl = (long long)g.px * g.py;
+  const uint32_t sample = *nk.sample;
+  const int second = 1 - first;
+  // groups holding no interior vertex (the x padding) draw no noise and update nothing
+  const bool inner = 8 * gx <= g.n0 - 1;
+  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
+  const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
+  const size_t aoff = (size_t)r * g.px;        // A rows j0-2 .. j0+TY+1 (chunked, LdsRow)
+  const size_t boff = (size_t)(r - 1) * g.px;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
+
+  // the march of chain(s) `chain` over tile `tile`, owned interior planes [ks, ke)
+  auto march = [&](const int tile, const int chain, const int ks, const int ke) {
   const int j0 = 1 + tile * TY;
   const int j = j0 - 2 + r;
-  const bool active = r < TY + 4;
   const bool has_mem = active && j >= 0 && j <= g.n1;
   const bool ring = active && j >= j0 - 1 && j <= j0 + TY && j >= 1 && j <= g.n1 - 1;
   const bool owned = active && j >= j0 && j <= j0 + TY - 1 && j <= g.n1 - 1;
-  const int ks = 1 + seg * SZ;
-  const int ke = ks + SZ < g.n2 ? ks + SZ : g.n2;  // owned interior planes [ks, ke)
-  const long long pl = (long long)g.px * g.py;
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
   const long long cbase1 = cbase + g.cs;  // second chain (CB = 2)
   const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
-  const uint32_t sample = *nk.sample;
-  const int second = 1 - first;
 
   T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
   T xmm1[8], xm1[8], xc1[8], xp1[8];
@@ -1365,13 +1367,6 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
     MGMC_RB3_INIT(xm1, xc1, pro1, chain + 1, cbase1)
   }
 #undef MGMC_RB3_INIT
-  // groups holding no interior vertex (the x padding) draw no noise and update nothing
-  const bool inner = 8 * gx <= g.n0 - 1;
-
-  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
-  const T om_d = (T)st.inv_diag_omega, ns = (T)st.noise_scale;
-  const size_t aoff = (size_t)r * g.px;        // A rows j0-2 .. j0+TY+1 (chunked, LdsRow)
-  const size_t boff = (size_t)(r - 1) * g.px;  // B rows j0-1 .. j0+TY (r = 1 .. TY+2)
 
   // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
   // receives plane k+1; fm, fc = f on planes k-1, k
@@ -1468,14 +1463,44 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
 #pragma unroll
     for (int q = 0; q < 8; ++q) fm[q] = fc[q];
   }
+  };
+
+  // nseg > 0: workgroup (chain group * nseg + seg) * ntile + tile marches the
+  // planes [1 + seg * SZ, 1 + (seg + 1) * SZ). nseg = 0: the ncol columns
+  // (chain group * ntile + tile) times the n2 - 1 interior planes, in that
+  // order, are split into equal runs, one per workgroup of the grid; a run
+  // that crosses into the next column is marched as two segments. The values
+  // written do not depend on where the segments end.
+  // (the host keeps ncol * (n2 - 1) below 2^31; the run bounds are uniform)
+  const int P = g.n2 - 1;
+  int a, e;
+  if (nseg > 0) {
+    const int b = (int)bk.x / ntile;
+    a = (b / nseg * ntile + (int)bk.x % ntile) * P + (b % nseg) * SZ;
+    e = a + SZ < (a / P + 1) * P ? a + SZ : (a / P + 1) * P;
+  } else {
+    const long long total = (long long)ncol * P;
+    a = (int)(total * bk.x / bk.gx);
+    e = (int)(total * (bk.x + 1) / bk.gx);
+  }
+  a = __builtin_amdgcn_readfirstlane(a);
+  e = __builtin_amdgcn_readfirstlane(e);
+  for (bool firstseg = true; a < e; firstseg = false) {
+    const int col = a / P;
+    const int p1 = e - col * P < P ? e - col * P : P;
+    if (!firstseg) __syncthreads();  // the previous segment's last LDS reads
+    march(col % ntile, col / ntile * CB, 1 + a - col * P, 1 + p1);
+    a = col * P + p1;
+  }
 }
 template <typename T, int NOISE, int CB>
 __global__ void __launch_bounds__((rb3_max_threads<T, CB>())) k_sweep_rb3(const T* __restrict__ x, T* __restrict__ xout,
                                                     const T* __restrict__ f, const T* __restrict__ ext, DevGeom g,
                                                     Stencil st, int first, int TY, int SZ, int ntile, int nseg,
-                                                    NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
+                                                    int ncol, NoiseKey nk, const T* __restrict__ xcoarse, DevGeom gc,
                                                     T alpha) {
-  b_sweep_rb3<T, NOISE, CB>(Blk::hw(), x, xout, f, ext, g, st, first, TY, SZ, ntile, nseg, nk, xcoarse, gc, alpha);
+  b_sweep_rb3<T, NOISE, CB>(Blk::hw(), x, xout, f, ext, g, st, first, TY, SZ, ntile, nseg, ncol, nk, xcoarse, gc,
+                            alpha);
 }
 template <typename T, int NOISE, int CB>
 constexpr Batchable<b_sweep_rb3<T, NOISE, CB>, (rb3_max_threads<T, CB>()), true, k_sweep_rb3<T, NOISE, CB>> kb_sweep_rb3{};
@@ -1774,12 +1799,16 @@ __global__ void __launch_bounds__(1024) k_sweep_box_march(const T* __restrict__ 
 // coarse z-segment [Ks, Ke): r(p) is formed from plane p in LDS and planes
 // p +- 1 in registers, restricted in x (left neighbour from LDS), in y (LDS)
 // and accumulated in z; coarse plane K is stored when fine plane 2K + 1 is
-// done. One thread per fine 8-vertex group and fine row.
+// done. One thread per fine 8-vertex group and fine row. Segments as in
+// k_sweep_rb3: nseg > 0 gives z-segments of SZc coarse planes, nseg = 0
+// splits the ncol columns (chain * ntile + tile) x coarse planes evenly over
+// the grid.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T* __restrict__ x, const T* __restrict__ f,
                                                                   T* __restrict__ fc, DevGeom g, DevGeom gc,
-                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg) {
+                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg,
+                                                                  int ncol) {
   MGMC_DYNAMIC_SHARED(char, lds_raw);
   const int nrx = 2 * TYc + 3, nrr = 2 * TYc + 1;
   T* LX = reinterpret_cast<T*>(lds_raw);    // nrx rows x px: x plane p
@@ -1788,26 +1817,22 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
   const int ngx = g.px >> 3;
   const int tid = threadIdx.x;
   const int rt = tid / ngx, gx = tid % ngx;
-  int b = bk.x;
-  const int tile = b % ntile;
-  b /= ntile;
-  const int seg = b % nseg;
-  const int chain = b / nseg;
+  const bool active = rt < nrx;
+  const bool rrow = active && rt >= 1 && rt <= nrr;                 // rows whose residual enters the tile
+  const long long pl = (long long)g.px * g.py;
+  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
+  const T half = (T)0.5;
+
+  // the march of chain `chain` over tile `tile`, coarse planes [Ks, Ke)
+  auto march = [&](const int tile, const int chain, const int Ks, const int Ke) {
   const int J0 = 1 + tile * TYc;
   const int j = 2 * J0 - 2 + rt;
-  const bool active = rt < nrx;
   const bool has_mem = active && j >= 0 && j <= g.n1;
-  const bool rrow = active && rt >= 1 && rt <= nrr;                 // rows whose residual enters the tile
   const bool rvalid = rrow && j >= 1 && j <= g.n1 - 1;
   const int J = j >> 1;
   const bool crow = rrow && !(j & 1) && J >= J0 && J < J0 + TYc && J <= gc.n1 - 1;
-  const int Ks = 1 + seg * SZc;
-  const int Ke = Ks + SZc < gc.n2 ? Ks + SZc : gc.n2;  // coarse planes [Ks, Ke)
-  const long long pl = (long long)g.px * g.py;
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
   const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
-  const T cx = (T)st.c[12], cy = (T)st.c[10], cz = (T)st.c[4], D = (T)st.diag;
-  const T half = (T)0.5;
 
   T xm[8], xc[8], xp[8], fb[8], fn[8];
 #pragma unroll
@@ -1887,12 +1912,36 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
       fb[q] = fn[q];
     }
   }
+  };
+
+  // (the host keeps ncol * (gc.n2 - 1) below 2^31; the run bounds are uniform)
+  const int P = gc.n2 - 1;
+  int a, e;
+  if (nseg > 0) {
+    const int b = (int)bk.x / ntile;
+    a = (b / nseg * ntile + (int)bk.x % ntile) * P + (b % nseg) * SZc;
+    e = a + SZc < (a / P + 1) * P ? a + SZc : (a / P + 1) * P;
+  } else {
+    const long long total = (long long)ncol * P;
+    a = (int)(total * bk.x / bk.gx);
+    e = (int)(total * (bk.x + 1) / bk.gx);
+  }
+  a = __builtin_amdgcn_readfirstlane(a);
+  e = __builtin_amdgcn_readfirstlane(e);
+  for (bool firstseg = true; a < e; firstseg = false) {
+    const int col = a / P;
+    const int p1 = e - col * P < P ? e - col * P : P;
+    if (!firstseg) __syncthreads();  // the previous segment's last LDS reads
+    march(col % ntile, col / ntile, 1 + a - col * P, 1 + p1);
+    a = col * P + p1;
+  }
 }
 template <typename T>
 __global__ void __launch_bounds__(rb3_max_threads<T>()) k_residual_restrict_star3(const T* __restrict__ x, const T* __restrict__ f,
                                                                   T* __restrict__ fc, DevGeom g, DevGeom gc,
-                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg) {
-  b_residual_restrict_star3<T>(Blk::hw(), x, f, fc, g, gc, st, TYc, SZc, ntile, nseg);
+                                                                  Stencil st, int TYc, int SZc, int ntile, int nseg,
+                                                                  int ncol) {
+  b_residual_restrict_star3<T>(Blk::hw(), x, f, fc, g, gc, st, TYc, SZc, ntile, nseg, ncol);
 }
 template <typename T>
 constexpr Batchable<b_residual_restrict_star3<T>, rb3_max_threads<T>(), true, k_residual_restrict_star3<T>> kb_residual_restrict_star3{};

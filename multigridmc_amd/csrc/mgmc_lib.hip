@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -210,6 +211,11 @@ struct Common {
   // (one FP64 chain at 128^3: 0.587 -> 0.549 ms per cycle; no change at 64)
   int tune_rb_min_planes = 2;
   int tune_rr_min_planes = 2;
+  // k_sweep_rb3 with its (chain, tile, plane) space split evenly over one
+  // round of resident workgroups instead of z-segments of rb_blocks (128^3 x
+  // 64 FP32 chains: 960 workgroups in 1.9 rounds -> 512 in one, cycle -2 %)
+  int tune_rb_balanced = 1;
+  int ncu = 1;  // compute units of the device
   int tune_lr_dot_chains = 1;     // chains per workgroup of k_lr_dot (1, 2, 4)
   int tune_lr_update_chains = 8;  // chains per thread of k_lr_update (2, 4, 8, 16)
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
@@ -367,9 +373,28 @@ struct Engine : EngineBase {
 
   // ---- fused red-black sweep (k_sweep_rb3) configuration ----
   struct RbShape {
-    int TY, SZ, ntile, nseg, threads, CB;
+    int TY, SZ, ntile, nseg, ncol, threads, CB;
     size_t lds;
+    unsigned grid;
   };
+  // resident workgroups per CU of a kernel at a launch shape (cached; not a
+  // stream operation, so also usable while a graph is captured)
+  std::map<std::tuple<const void*, int, size_t>, int> occupancy_cache;
+  int resident_per_cu(const void* kern, int threads, size_t lds) {
+#ifndef MGMC_EMULATION
+    auto key = std::make_tuple(kern, threads, lds);
+    auto it = occupancy_cache.find(key);
+    if (it != occupancy_cache.end()) return it->second;
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+    per_cu = std::max(1, per_cu);
+    occupancy_cache[key] = per_cu;
+    return per_cu;
+#else
+    (void)kern, (void)threads, (void)lds;
+    return 2;
+#endif
+  }
   // chains per workgroup: 2 when asked for, f is shared by the chains and K is even
   int rb_cb(int l) const {
     return sizeof(T) == 4 && C.tune_rb_chains == 2 && geo(l).fcs == 0 && C.K % 2 == 0 ? 2 : 1;
@@ -388,9 +413,12 @@ struct Engine : EngineBase {
   bool fused_ok(int l) const {
     const Level& Lv = *C.L[l];
     if (lex() || Lv.variable || !C.tune_fused || !Lv.star || Lv.g.dim != 3) return false;
-    return rb_ty(l) >= std::min(4, Lv.g.n1 - 1);
+    const int ty = rb_ty(l);
+    if (ty < std::min(4, Lv.g.n1 - 1)) return false;
+    // the kernel numbers its (chain, tile, plane) units in 32 bits
+    return (long long)C.K * ((Lv.g.n1 - 1 + ty - 1) / ty) * (Lv.g.n2 - 1) < (1LL << 31);
   }
-  RbShape rb_shape(int l) const {
+  RbShape rb_shape(int l, int noise) {
     const DevGeom& g = C.L[l]->g;
     RbShape s;
     int ngx = g.px >> 3;
@@ -405,7 +433,26 @@ struct Engine : EngineBase {
     s.nseg = (nplanes + s.SZ - 1) / s.SZ;
     s.threads = ((ngx * (s.TY + 4) + 63) / 64) * 64;
     s.lds = ((size_t)2 * (2 * s.TY + 6) * g.px + 5 * 8) * sizeof(T) * s.CB;  // + the zero guards of k_sweep_rb3
+    s.ncol = s.ntile * ngroups;
+    s.grid = (unsigned)(s.ntile * s.nseg * ngroups);
+    if (C.tune_rb_balanced) {
+      const long long units = (long long)s.ncol * nplanes;
+      const long long slots = (long long)C.ncu * resident_per_cu(rb_kernel(noise, s.CB), s.threads, s.lds);
+      s.nseg = 0;
+      s.grid = (unsigned)std::max(1LL, std::min(slots, units / C.tune_rb_min_planes));
+    }
     return s;
+  }
+  static const void* rb_kernel(int noise, int cb) {
+    if constexpr (sizeof(T) == 4) {
+      if (cb == 2)
+        return noise == NOISE_NONE     ? reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE, 2>)
+               : noise == NOISE_PHILOX ? reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX, 2>)
+                                       : reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL, 2>);
+    }
+    return noise == NOISE_NONE     ? reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_NONE, 1>)
+           : noise == NOISE_PHILOX ? reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_PHILOX, 1>)
+                                   : reinterpret_cast<const void*>(k_sweep_rb3<T, NOISE_EXTERNAL, 1>);
   }
   // prolongate-add pending for the next fused sweep of a level (folded into its loads)
   std::vector<double> pending_alpha;
@@ -413,11 +460,11 @@ struct Engine : EngineBase {
 
   void fused_sweep(int l, bool forward, int noise, const T* extp, uint32_t tag) {
     Level& Lv = *C.L[l];
-    RbShape s = rb_shape(l);
+    RbShape s = rb_shape(l, noise);
     T* in = X(l);
     T* out = cur[l] ? x[l].p : x2[l].p;
     NoiseKey nk = key(tag);
-    dim3 grid((unsigned)(s.ntile * s.nseg * (C.K / s.CB)));
+    dim3 grid(s.grid);
     int first = forward ? 0 : 1;
     const bool timed = prof_begin(l);
     const T* xcoarse = nullptr;
@@ -431,7 +478,7 @@ struct Engine : EngineBase {
     }
 #define MGMC_RB3(NOISE, CB)                                                                                      \
   launch(grid, s.threads, s.lds, C.stream, kb_sweep_rb3<T, NOISE, CB>, in, out, f[l].p, extp, geo(l), Lv.st, first, \
-         s.TY, s.SZ, s.ntile, s.nseg, nk, xcoarse, gc, alpha)
+         s.TY, s.SZ, s.ntile, s.nseg, s.ncol, nk, xcoarse, gc, alpha)
     if constexpr (sizeof(T) == 4) {  // chain-blocked FP32 only (rb_cb)
       if (s.CB == 2) {
         if (noise == NOISE_NONE) MGMC_RB3(NOISE_NONE, 2);
@@ -878,7 +925,11 @@ struct Engine : EngineBase {
   bool rr_ok(int l) const {
     const Level& Lv = *C.L[l];
     if (Lv.variable || !C.tune_fused_rr || !Lv.star || Lv.g.dim != 3) return false;
-    return rr_tyc(l) >= 1;
+    const int t = rr_tyc(l);
+    if (t < 1) return false;
+    // the kernel numbers its (chain, tile, coarse plane) units in 32 bits
+    const DevGeom& gc = C.L[l + 1]->g;
+    return (long long)C.K * ((gc.n1 - 1 + t - 1) / t) * (gc.n2 - 1) < (1LL << 31);
   }
   int rr_tyc(int l) const {
     const DevGeom& g = C.L[l]->g;
@@ -907,9 +958,17 @@ struct Engine : EngineBase {
     const int ngx = g.px >> 3;
     const int threads = ((ngx * (2 * TYc + 3) + 63) / 64) * 64;
     const size_t lds = ((size_t)(4 * TYc + 4) * g.px + (size_t)(2 * TYc + 1) * (g.px >> 1)) * sizeof(T);
-    dim3 grid((unsigned)(ntile * nseg * C.K));
-    launch(grid, threads, lds, C.stream, kb_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, geo(l), gc, F.st, TYc,
-           SZc, ntile, nseg);
+    const int ncol = ntile * C.K;
+    unsigned grid = (unsigned)(ntile * nseg * C.K);
+    if (C.tune_rb_balanced) {  // one round of resident workgroups, as the level sweep
+      const long long units = (long long)ncol * ncpl;
+      const long long slots =
+          (long long)C.ncu * resident_per_cu(reinterpret_cast<const void*>(k_residual_restrict_star3<T>), threads, lds);
+      nseg = 0;
+      grid = (unsigned)std::max(1LL, std::min(slots, units / C.tune_rr_min_planes));
+    }
+    launch(dim3(grid), threads, lds, C.stream, kb_residual_restrict_star3<T>, X(l), f[l].p, f[l + 1].p, geo(l), gc, F.st,
+           TYc, SZc, ntile, nseg, ncol);
   }
 
   // r = f - A x on a variable level (precision U: the cycle's, or FP64 for the loop solver)
@@ -1334,8 +1393,10 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
 #ifndef MGMC_EMULATION
   HIP_CHECK(hipGetDevice(&C.device));
   HIP_CHECK(hipDeviceGetAttribute(&C.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, C.device));
+  HIP_CHECK(hipDeviceGetAttribute(&C.ncu, hipDeviceAttributeMultiprocessorCount, C.device));
 #else
   C.max_lds = 163840;
+  C.ncu = 3;  // emulation: a few "CUs", so that balanced runs cross columns
 #endif
   C.scratch = scratch_address(C.device);
   for (int l = 0; l < C.nlevel; ++l) {
@@ -2360,6 +2421,8 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
   } else if (k == "rb_chains") {
     if (value != 1 && value != 2) throw Error(MGMC_ERR_ARG, "rb_chains must be 1 or 2");
     h->C.tune_rb_chains = value;
+  } else if (k == "rb_balanced") {
+    h->C.tune_rb_balanced = value != 0;
   } else
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
@@ -2378,6 +2441,7 @@ static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_threads") return h->C.tune_rb_threads;
   if (k == "rb_blocks") return h->C.tune_rb_blocks;
   if (k == "rb_chains") return h->C.tune_rb_chains;
+  if (k == "rb_balanced") return h->C.tune_rb_balanced;
   if (k == "lr_dot_chains") return h->C.tune_lr_dot_chains;
   if (k == "rb_min_planes") return h->C.tune_rb_min_planes;
   if (k == "rr_min_planes") return h->C.tune_rr_min_planes;
@@ -2436,6 +2500,7 @@ int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms) {
     std::vector<int> values;
   };
   std::vector<Key> keys = {{"rb_chains", {1, 2}},
+                           {"rb_balanced", {0, 1}},
                            {"rb_threads", {256, 384, 512, 768, 1024}},
                            {"rb_blocks", {384, 512, 768, 1024, 1536, 2048}},
                            {"rb_min_planes", {1, 2, 4, 8}},

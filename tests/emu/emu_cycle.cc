@@ -269,6 +269,50 @@ void batch_case() {
   }
 }
 
+// The level-0 red-black sweep with its (chain, tile, plane) space split
+// evenly over the resident workgroups (rb_balanced: runs that cross from one
+// column into the next) must give the samples of its z-segment form.
+void balanced_case() {
+  std::printf("case balanced level-0 sweep split\n");
+  std::fflush(stdout);
+  for (int precision : {MGMC_FP32, MGMC_FP64}) {
+    // 8 rows per tile at 64 threads: 2 tiles x 4 chains = 8 columns of 19
+    // planes over 6 emulated resident workgroups
+    const Case c = {"40x12x20", 3, {40, 12, 20}, 2, 2, precision, MGMC_ORDER_MULTICOLOUR, 4, false};
+    ProblemData d;
+    build(c, d);
+    const int64_t N = d.N;
+    std::vector<double> f((size_t)N), x((size_t)(N * c.K), 0.0), st[2], obs[2];
+    for (int64_t v = 0; v < N; ++v) f[(size_t)v] = std::sin(0.21 * (double)v);
+    const int64_t probes[2] = {3, N / 2};
+    for (int bal = 0; bal < 2; ++bal) {
+      mgmc_handle* h = nullptr;
+      check(mgmc_create(&d.p, &d.par, &h), "mgmc_create");
+      check(mgmc_set_tuning(h, "rb_threads", 64), "mgmc_set_tuning");
+      check(mgmc_set_tuning(h, "rb_balanced", bal), "mgmc_set_tuning");
+      int got = -1;
+      check(mgmc_get_tuning(h, "rb_balanced", &got), "mgmc_get_tuning");
+      if (got != bal) {
+        std::printf("FAIL balanced: rb_balanced reads %d\n", got);
+        ++failures;
+      }
+      check(mgmc_set_rhs(h, f.data(), 0), "mgmc_set_rhs");
+      check(mgmc_set_state(h, x.data()), "mgmc_set_state");
+      check(mgmc_set_probes(h, 2, probes), "mgmc_set_probes");
+      obs[bal].assign((size_t)(3 * c.K * 2), 0.0);
+      check(mgmc_run(h, 3, obs[bal].data(), 0), "mgmc_run");
+      st[bal].assign((size_t)(N * c.K), 0.0);
+      check(mgmc_get_state(h, st[bal].data()), "mgmc_get_state");
+      check(mgmc_destroy(h), "mgmc_destroy");
+    }
+    if (std::memcmp(st[0].data(), st[1].data(), st[0].size() * sizeof(double)) != 0 ||
+        std::memcmp(obs[0].data(), obs[1].data(), obs[0].size() * sizeof(double)) != 0) {
+      std::printf("FAIL balanced: %s samples differ from the z-segment split\n", precision == MGMC_FP32 ? "fp32" : "fp64");
+      ++failures;
+    }
+  }
+}
+
 }  // namespace
 
 int main() {
@@ -282,6 +326,7 @@ int main() {
   };
   for (const Case& c : cases) run(c);
   batch_case();
+  balanced_case();
   if (failures) return 1;
   std::printf("emu_cycle ok\n");
   return 0;

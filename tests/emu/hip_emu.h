@@ -63,6 +63,7 @@ inline double2 make_double2(double a, double b) { return double2{a, b}; }
 inline float __builtin_amdgcn_logf(float v) { return std::log2(v); }
 inline float __builtin_amdgcn_sinf(float v) { return std::sin(6.283185307179586f * v); }
 inline float __builtin_amdgcn_cosf(float v) { return std::cos(6.283185307179586f * v); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }  // the values it is used on are uniform
 
 // ---- runtime API ----
 typedef enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorNotSupported = 801 } hipError_t;

@@ -27,6 +27,7 @@ def test_autotune_keeps_state_and_sample_path(precision):
     # start the search away from the defaults
     tuned.set_tuning("rb_threads", 256)
     tuned.set_tuning("rb_blocks", 2048)
+    tuned.set_tuning("rb_balanced", 0)
     tuned.set_tuning("plane_sweep_stage3", 0)
     before = tuned.get_state()
     ms = tuned.autotune(ncycles=2)

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import multigridmc_amd as mg
+from multigridmc_amd import measurements as ms
 from gpu_common import make_problem, normals_reference, oracle_mg, params
 from oracle import pyoracle as po
 
@@ -213,11 +214,14 @@ def test_csr_operator_input_matches_fd():
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
-@pytest.mark.parametrize("threads,blocks", [(512, 1024), (128, 1), (64, 100000)])
-def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
+@pytest.mark.parametrize("threads,blocks,balanced", [(512, 1024, 0), (128, 1, 0), (64, 100000, 0), (512, 1, 1),
+                                                     (64, 1, 1)])
+def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks, balanced):
     """The single-pass red-black kernel (tiles of full x-rows, z-segments with
     redundant halo updates) equals two colour-phase launches, for Philox and
-    injected noise, in both directions, over several tile / segment shapes."""
+    injected noise, in both directions, over several tile / segment shapes,
+    also with the (chain, tile, plane) space split evenly over the resident
+    workgroups (runs that cross from one column into the next)."""
     op, _ = make_problem(3, (32, 16, 24), 3)
     p = params(3)
     K = 3
@@ -226,6 +230,7 @@ def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
     fus = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=5)
     fus.set_tuning("rb_threads", threads)
     fus.set_tuning("rb_blocks", blocks)
+    fus.set_tuning("rb_balanced", balanced)
     N = op.lattice.Nvertex
     rng = np.random.default_rng(6)
     b, x, xi = rng.standard_normal((3, K, N))
@@ -238,6 +243,31 @@ def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks):
                 a = ref.sweep(0, fw, st, b, x, **kw)
                 c = fus.sweep(0, fw, st, b, x, **kw)
                 assert np.abs(a - c).max() <= tol * np.abs(a).max(), (fw, st, bool(kw))
+
+
+@pytest.mark.parametrize("precision,n,K,threads", [("fp32", (128, 128, 128), 64, 512), ("fp64", (128, 128, 128), 1, 512),
+                                                   ("fp32", (48, 40, 36), 5, 64), ("fp64", (48, 40, 36), 3, 128)])
+def test_balanced_sweep_split_gives_the_same_samples(precision, n, K, threads):
+    """Whole sampling cycles (folded coarse correction, low-rank terms) with
+    the level-0 sweep's (chain, tile, plane) space split evenly over one round
+    of resident workgroups equal those of its z-segment split bit for bit,
+    among them the 128^3 posterior with 32 measurements and 64 FP32 chains."""
+    lat = mg.Lattice(*n)
+    centres, y, var = mg.seeded_ball_measurements(lat, 32, 0.05, 1e-4, 3)
+    op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
+    p = mg.MultigridParameters(nlevel=ms._nlevel(op))
+    f = op.get_B() @ (y / var)
+    states = []
+    for balanced in (0, 1):
+        s = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=44)
+        s.set_tuning("rb_threads", threads)
+        s.set_tuning("rb_balanced", balanced)
+        assert s.get_tuning("rb_balanced") == balanced
+        s.fix_rhs(f)
+        s.set_state(np.zeros((K, lat.Nvertex)))
+        s.run(3, record=False)
+        states.append(s.get_state())
+    assert np.array_equal(states[0], states[1])
 
 
 @pytest.mark.parametrize("n,K", [((32, 16, 24), 4), ((24, 20, 16), 6)])
