@@ -261,6 +261,14 @@ struct ProlongStream {
   }
   // v += alpha (P xc) on fine plane p (p never below the previous call's)
   __device__ __forceinline__ void add(T* v, int p) {
+    advance(p);
+    apply(v, p);
+  }
+  // the two halves of add: advance(p) requests the coarse rows fine plane p
+  // needs (when the thread requests plane p of x), apply(v, p) adds the
+  // correction (when plane p is used), so that both loads are in flight
+  // meanwhile; between them no other plane may be advanced to
+  __device__ __forceinline__ void advance(int p) {
     if (!cb) return;
     if ((p >> 1) != kl) {
       ++kl;
@@ -268,6 +276,9 @@ struct ProlongStream {
       for (int e = 0; e < 5; ++e) sl[e] = sh[e];
       row_values(kl + 1, sh);
     }
+  }
+  __device__ __forceinline__ void apply(T* v, int p) const {
+    if (!cb) return;
     T t[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) t[e] = (p & 1) ? (T)0.5 * (sl[e] + sh[e]) : sl[e];
@@ -1376,10 +1387,13 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
     const bool storeB = owned && kb >= ks && kb <= ke - 1;
     const bool doB = storeB && inner;
     // loads first: next plane of x, f on plane k
+    // (the coarse rows of plane k+1's correction are requested with it and
+    // added after the noise, before the first colour reads plane k+1)
+    const bool next = has_mem && k + 1 <= g.n2;
 #define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
-  if (has_mem && k + 1 <= g.n2) {                         \
+  if (next) {                                             \
     load8<T>(x + CBASE + (long long)(k + 1) * pl, XP);    \
-    PRO.add(XP, k + 1);                                   \
+    PRO.advance(k + 1);                                   \
   } else {                                                \
     _Pragma("unroll") for (int q = 0; q < 8; ++q) XP[q] = (T)0; \
   }
@@ -1403,6 +1417,10 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
         if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiA1);
         if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiB1);
       }
+    }
+    if (next) {
+      pro.apply(xp, k + 1);
+      if constexpr (CB > 1) pro1.apply(xp1, k + 1);
     }
     if (active) LdsRow<T>::store(A0 + (k & 1) * asz + aoff, gx, ngx, xc);  // plane k before the first colour
     if constexpr (CB > 1) {

@@ -271,7 +271,9 @@ void batch_case() {
 
 // The level-0 red-black sweep with its (chain, tile, plane) space split
 // evenly over the resident workgroups (rb_balanced: runs that cross from one
-// column into the next) must give the samples of its z-segment form.
+// column into the next) must give the samples of its z-segment form, and
+// with the coarse correction added while the first post-sweep reads its
+// planes, those of a separate prolongation launch (to rounding).
 void balanced_case() {
   std::printf("case balanced level-0 sweep split\n");
   std::fflush(stdout);
@@ -282,14 +284,16 @@ void balanced_case() {
     ProblemData d;
     build(c, d);
     const int64_t N = d.N;
-    std::vector<double> f((size_t)N), x((size_t)(N * c.K), 0.0), st[2], obs[2];
+    std::vector<double> f((size_t)N), x((size_t)(N * c.K), 0.0), st[3], obs[3];
     for (int64_t v = 0; v < N; ++v) f[(size_t)v] = std::sin(0.21 * (double)v);
     const int64_t probes[2] = {3, N / 2};
-    for (int bal = 0; bal < 2; ++bal) {
+    for (int v = 0; v < 3; ++v) {
+      const int bal = v == 0 ? 0 : 1;
       mgmc_handle* h = nullptr;
       check(mgmc_create(&d.p, &d.par, &h), "mgmc_create");
       check(mgmc_set_tuning(h, "rb_threads", 64), "mgmc_set_tuning");
       check(mgmc_set_tuning(h, "rb_balanced", bal), "mgmc_set_tuning");
+      if (v == 2) check(mgmc_set_tuning(h, "fold_prolongation", 0), "mgmc_set_tuning");
       int got = -1;
       check(mgmc_get_tuning(h, "rb_balanced", &got), "mgmc_get_tuning");
       if (got != bal) {
@@ -299,15 +303,26 @@ void balanced_case() {
       check(mgmc_set_rhs(h, f.data(), 0), "mgmc_set_rhs");
       check(mgmc_set_state(h, x.data()), "mgmc_set_state");
       check(mgmc_set_probes(h, 2, probes), "mgmc_set_probes");
-      obs[bal].assign((size_t)(3 * c.K * 2), 0.0);
-      check(mgmc_run(h, 3, obs[bal].data(), 0), "mgmc_run");
-      st[bal].assign((size_t)(N * c.K), 0.0);
-      check(mgmc_get_state(h, st[bal].data()), "mgmc_get_state");
+      obs[v].assign((size_t)(3 * c.K * 2), 0.0);
+      check(mgmc_run(h, 3, obs[v].data(), 0), "mgmc_run");
+      st[v].assign((size_t)(N * c.K), 0.0);
+      check(mgmc_get_state(h, st[v].data()), "mgmc_get_state");
       check(mgmc_destroy(h), "mgmc_destroy");
     }
     if (std::memcmp(st[0].data(), st[1].data(), st[0].size() * sizeof(double)) != 0 ||
         std::memcmp(obs[0].data(), obs[1].data(), obs[0].size() * sizeof(double)) != 0) {
       std::printf("FAIL balanced: %s samples differ from the z-segment split\n", precision == MGMC_FP32 ? "fp32" : "fp64");
+      ++failures;
+    }
+    double dmax = 0.0, amax = 0.0;
+    for (size_t i = 0; i < st[1].size(); ++i) {
+      dmax = std::max(dmax, std::fabs(st[1][i] - st[2][i]));
+      amax = std::max(amax, std::fabs(st[2][i]));
+    }
+    const double tol = precision == MGMC_FP64 ? 1e-12 : 2e-5;
+    if (!(dmax <= tol * amax)) {
+      std::printf("FAIL fold: %s folded coarse correction differs from the separate launch by %.3e (max |x| %.3e)\n",
+                  precision == MGMC_FP32 ? "fp32" : "fp64", dmax, amax);
       ++failures;
     }
   }
