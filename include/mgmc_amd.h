@@ -327,8 +327,10 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
  * "rr_min_planes" (fewest planes per z-segment of the fused level sweep /
  * residual-restriction), "lr_dot_chains" (1, 2, 4:
  * chains per workgroup of the low-rank dot B^T x), "lr_update_chains" (2, 4,
- * 8, 16: chains per thread of the low-rank update, with more than 8 chains).
- * Results do not depend on these settings beyond rounding. */
+ * 8, 16: chains per thread of the low-rank update, with more than 8 chains),
+ * "lr_carry" (0/1, default 1: the residual's B^T x taken from the SMW step
+ * just before it, B^T (y + W u) = B^T y + (B^T W) u, instead of a second
+ * gather over x). Results do not depend on these settings beyond rounding. */
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
 /* Autotuner (no reference counterpart): coordinate search over the tuning
@@ -340,7 +342,7 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
  * so the sample path after tuning equals the untuned one bit for bit.
  * best_ms (may be NULL) receives the best time per cycle. */
 int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms);
-/* Current value of one of the autotuned keys. */
+/* Current value of one of the autotuned keys, or of "lr_carry". */
 int mgmc_get_tuning(const mgmc_handle* h, const char* key, int* value);
 
 /* Profiling aid: with enable = 1 every level-0 sweep launch (one fused

@@ -222,29 +222,27 @@ struct ProlongStream {
   T alpha;
   T sl[5], sh[5];
 
+  // s = the mean of coarse rows j0 and j1 of plane K (one row when j0 = j1:
+  // (v + v) / 2 = v exactly); both rows' loads are issued together
   __device__ __forceinline__ void row_values(int K, T* s) const {
 #pragma unroll
     for (int e = 0; e < 5; ++e) s[e] = (T)0;
     if (K > gc.n2) return;  // beyond the last coarse plane: boundary, zero
-    const int nr = j1 != j0 ? 2 : 1;
-    for (int b = 0; b < nr; ++b) {
+    T v[2][5];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
       const T* p = cb + ((long long)K * gc.py + (b ? j1 : j0)) * gc.px + 4 * gx;
-      T v[4];
       if constexpr (sizeof(T) == 4) {
         const float4 a = *reinterpret_cast<const float4*>(p);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[b][0] = a.x; v[b][1] = a.y; v[b][2] = a.z; v[b][3] = a.w;
       } else {
         const double2 a = reinterpret_cast<const double2*>(p)[0], c = reinterpret_cast<const double2*>(p)[1];
-        v[0] = a.x; v[1] = a.y; v[2] = c.x; v[3] = c.y;
+        v[b][0] = a.x; v[b][1] = a.y; v[b][2] = c.x; v[b][3] = c.y;
       }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) s[e] += v[e];
-      s[4] += 4 * gx + 4 < gc.px ? p[4] : (T)0;
+      v[b][4] = 4 * gx + 4 < gc.px ? p[4] : (T)0;
     }
-    if (nr == 2) {
 #pragma unroll
-      for (int e = 0; e < 5; ++e) s[e] *= (T)0.5;
-    }
+    for (int e = 0; e < 5; ++e) s[e] = (T)0.5 * (v[0][e] + v[1][e]);
   }
   // first fine plane this thread will visit
   __device__ __forceinline__ void init(const T* coarse, const DevGeom& g_c, int gx_, int j, int p0, T a) {
@@ -678,17 +676,61 @@ __global__ void __launch_bounds__(256) k_lr_dot(const T* __restrict__ x, long lo
 template <typename T, int LR_CPB>
 constexpr Batchable<b_lr_dot<T, LR_CPB>, 256, false, k_lr_dot<T, LR_CPB>> kb_lr_dot{};
 
-// one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c)
+// B^T x for an x that is y + W u_p, the state right after an SMW step of
+// the same level (y the sweep's output, z_p = B^T y the partial sums in z,
+// u_p = M_p (sqrt(Sigma) xi'_p - z_p) the step's solve, W = T^{-1} B):
+// B^T x = z_p + G u_p with G = B^T W (m x m), so the residual's low-rank term
+// needs no second pass over x. G = nullptr: z holds B^T x itself.
+struct LrCarry {
+  const double* G;
+  const double* Mp;    // the step's M
+  int noise;           // its low-rank noise (NOISE_*), key and external values
+  NoiseKey nk;
+  const double* ext;
+};
+
+// sqrt(Sigma_i) xi'_i of chain c: the low-rank noise of an SMW sampler step
+__device__ __forceinline__ double lr_noise(int noise, int i, int c, const NoiseKey& nk,
+                                           const double* __restrict__ sqrt_sigma, const double* __restrict__ ext, int m) {
+  if (noise == NOISE_PHILOX) {
+    float xi[4];
+    normals4(LOWRANK_SLOT + (uint32_t)(i >> 2), (uint32_t)c, nk.tag, *nk.sample, nk.k0, nk.k1, xi);
+    return sqrt_sigma[i] * (double)xi[i & 3];
+  }
+  if (noise == NOISE_EXTERNAL) return sqrt_sigma[i] * ext[(long long)c * m + i];
+  return 0.0;
+}
+
+// one block (MMAX threads) per chain: u_c = M (a sqrt(Sigma) xi'_c + zc z_c),
+// z_c from the partial sums, or with a carry z_c = z_p + G u_p (LrCarry)
 template <int NOISE>
 __device__ __forceinline__ void b_lr_solve(const Blk bk, const double* __restrict__ z, const double* __restrict__ M,
                                                    const double* __restrict__ sqrt_sigma, double zc,
                                                    const double* __restrict__ ext_lr, double* __restrict__ u, int m,
-                                                   NoiseKey nk, int nsplit) {
-  __shared__ double a[MMAX];
+                                                   NoiseKey nk, int nsplit, LrCarry cy) {
+  __shared__ double a[MMAX], t[MMAX];
   const int c = bk.x, tid = threadIdx.x;
-  if (tid < m) {
-    double zs = 0.0;
+  double zs = 0.0;
+  if (tid < m)
     for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + tid) * nsplit + p];
+  if (cy.G) {
+    // u_p as the step's own solve formed it, then z_p + G u_p
+    if (tid < m) a[tid] = -zs + lr_noise(cy.noise, tid, c, cy.nk, sqrt_sigma, cy.ext, m);
+    __syncthreads();
+    if (tid < m) {
+      double acc = 0.0;
+      for (int l = 0; l < m; ++l) acc += cy.Mp[tid * m + l] * a[l];
+      t[tid] = acc;
+    }
+    __syncthreads();
+    if (tid < m) {
+      double acc = zs;
+      for (int l = 0; l < m; ++l) acc += cy.G[tid * m + l] * t[l];
+      zs = acc;
+    }
+    __syncthreads();  // every a[l] is read before it is overwritten
+  }
+  if (tid < m) {
     double v = zc * zs;
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
@@ -709,8 +751,8 @@ template <int NOISE>
 __global__ void __launch_bounds__(MMAX) k_lr_solve(const double* __restrict__ z, const double* __restrict__ M,
                                                    const double* __restrict__ sqrt_sigma, double zc,
                                                    const double* __restrict__ ext_lr, double* __restrict__ u, int m,
-                                                   NoiseKey nk, int nsplit) {
-  b_lr_solve<NOISE>(Blk::hw(), z, M, sqrt_sigma, zc, ext_lr, u, m, nk, nsplit);
+                                                   NoiseKey nk, int nsplit, LrCarry cy) {
+  b_lr_solve<NOISE>(Blk::hw(), z, M, sqrt_sigma, zc, ext_lr, u, m, nk, nsplit, cy);
 }
 template <int NOISE>
 constexpr Batchable<b_lr_solve<NOISE>, MMAX, false, k_lr_solve<NOISE>> kb_lr_solve{};
@@ -781,7 +823,7 @@ __device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ 
                                                          const double* __restrict__ z, const double* __restrict__ M,
                                                          const double* __restrict__ sqrt_sigma, double zc,
                                                          const double* __restrict__ ext_lr, int m, int K, NoiseKey nk,
-                                                         int nsplit) {
+                                                         int nsplit, LrCarry cy) {
   __shared__ double a[LR_CPU][MMAX];
   __shared__ double us[LR_CPU][MMAX];
   const int c0 = bk.y * LR_CPU;
@@ -799,10 +841,32 @@ __device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ 
   T xv[LR_CPU];
 #pragma unroll
   for (int q = 0; q < LR_CPU; ++q) xv[q] = (ok && q < nc) ? x[(long long)(c0 + q) * cs + rr] : (T)0;
+  if (cy.G) {
+    // with a carry (LrCarry): u_p of the SMW step before, in us
+    for (int e = tid; e < nc * m; e += blockDim.x) {
+      const int q = e / m, i = e % m, c = c0 + q;
+      double zs = 0.0;
+      for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + i) * nsplit + p];
+      a[q][i] = -zs + lr_noise(cy.noise, i, c, cy.nk, sqrt_sigma, cy.ext, m);
+    }
+    __syncthreads();
+    for (int e = tid; e < nc * m; e += blockDim.x) {
+      const int q = e / m, i = e % m;
+      double acc = 0.0;
+      for (int l = 0; l < m; ++l) acc += cy.Mp[i * m + l] * a[q][l];
+      us[q][i] = acc;
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < nc * m; e += blockDim.x) {
     const int q = e / m, i = e % m, c = c0 + q;
     double zs = 0.0;
     for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + i) * nsplit + p];
+    if (cy.G) {
+      double acc = zs;
+      for (int l = 0; l < m; ++l) acc += cy.G[i * m + l] * us[q][l];
+      zs = acc;
+    }
     double v = zc * zs;
     if (NOISE == NOISE_PHILOX) {
       float xi[4];
@@ -845,8 +909,9 @@ __global__ void __launch_bounds__(256) k_lr_solve_update(T* __restrict__ x, long
                                                          const double* __restrict__ z, const double* __restrict__ M,
                                                          const double* __restrict__ sqrt_sigma, double zc,
                                                          const double* __restrict__ ext_lr, int m, int K, NoiseKey nk,
-                                                         int nsplit) {
-  b_lr_solve_update<T, NOISE>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, z, M, sqrt_sigma, zc, ext_lr, m, K, nk, nsplit);
+                                                         int nsplit, LrCarry cy) {
+  b_lr_solve_update<T, NOISE>(Blk::hw(), x, cs, rows, ptr, col, val, nrows, z, M, sqrt_sigma, zc, ext_lr, m, K, nk, nsplit,
+                              cy);
 }
 template <typename T, int NOISE>
 constexpr Batchable<b_lr_solve_update<T, NOISE>, 256, false, k_lr_solve_update<T, NOISE>> kb_lr_solve_update{};

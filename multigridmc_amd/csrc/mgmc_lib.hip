@@ -159,6 +159,7 @@ struct Level {
   DevRows Brows;
   DevRows W[2];
   DevBuf<double> Sinv[2];
+  DevBuf<double> G[2];        // B^T W per direction (residual terms carried from an SMW step)
   DevBuf<double> SinvLex[2];  // capacitance inverses for the lexicographic order (mgmc_set_ordering)
   // per-vertex stencils (variable coefficients): nplanes planes of the padded box
   bool variable = false;
@@ -218,6 +219,9 @@ struct Common {
   int ncu = 1;  // compute units of the device
   int tune_lr_dot_chains = 1;     // chains per workgroup of k_lr_dot (1, 2, 4)
   int tune_lr_update_chains = 8;  // chains per thread of k_lr_update (2, 4, 8, 16)
+  // the residual's B^T x from the SMW step just before it (z_p + B^T W u_p)
+  // instead of a second gather over x
+  int tune_lr_carry = 1;
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
   int tune_march_blocks = 512; // ... workgroups they aim for
   int tune_march_min_planes = 4; // ... fewest planes per z-segment
@@ -745,10 +749,50 @@ struct Engine : EngineBase {
     const double* extlr = nullptr;
     uint32_t tag = 0;
     int nsplit = 1;  // partial sums per entry of z
+    LrCarry carry{};  // z is B^T x itself (carry.G = nullptr) or z_p of a carried step
   } lr_next;
+  // Level whose x is exactly the output of its last SMW step (the sweep's
+  // state plus W u, with z, the noise and M of that step in lr_carry), or -1:
+  // every other write of a level's x and every other low-rank solve ends it.
+  int carry_l = -1;
+  LrSolve lr_carry;
+  LrCarry carry_of(int l) const {
+    const LrSolve& p = lr_carry;
+    return LrCarry{C.L[l]->G[carry_dir].p, p.M, p.noise, key(p.tag), p.extlr};
+  }
+  int carry_dir = 0;
+  void end_carry(int l) {
+    if (carry_l == l || l < 0) carry_l = -1;
+  }
   bool lr_fused() const { return C.K <= LR_CPU; }
+  void lr_solve_launch(const double* M, int noise, const double* extlr, uint32_t tag, int nsplit, LrCarry cy) {
+    NoiseKey nk = key(tag);
+    if (noise == NOISE_NONE)
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+             nsplit, cy);
+    else if (noise == NOISE_PHILOX)
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
+             nsplit, cy);
+    else
+      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
+             nk, nsplit, cy);
+  }
+  // u = M (-B_l^T x) for the residual's low-rank term (M = Sigma^{-1}): from
+  // the carried step when x is still its output, else k_lr_dot + solve
+  void lr_residual_solve(int l, const T* xp) {
+    if (carry_l != l || !C.tune_lr_carry) {
+      lr_dot_solve(l, xp, C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+      return;
+    }
+    const LrCarry cy = carry_of(l);
+    carry_l = -1;
+    lr_next = LrSolve{C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0, lr_carry.nsplit, cy};
+    if (lr_fused()) return;
+    lr_solve_launch(C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0, lr_carry.nsplit, cy);
+  }
   void lr_dot_solve(int l, const T* xp, const double* M, int noise, const double* extlr, uint32_t tag) {
     Level& Lv = *C.L[l];
+    carry_l = -1;
     const int cpb = C.tune_lr_dot_chains;
     dim3 grid(C.m * Lv.nsplit, (C.K + cpb - 1) / cpb);
 #define MGMC_LRDOT(CPB) \
@@ -758,18 +802,9 @@ struct Engine : EngineBase {
     else if (cpb == 2) MGMC_LRDOT(2);
     else MGMC_LRDOT(1);
 #undef MGMC_LRDOT
-    lr_next = LrSolve{M, noise, extlr, tag, Lv.nsplit};
+    lr_next = LrSolve{M, noise, extlr, tag, Lv.nsplit, LrCarry{}};
     if (lr_fused()) return;
-    NoiseKey nk = key(tag);
-    if (noise == NOISE_NONE)
-      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_NONE>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
-             Lv.nsplit);
-    else if (noise == NOISE_PHILOX)
-      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_PHILOX>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m, nk,
-             Lv.nsplit);
-    else
-      launch(C.K, MMAX, 0, C.stream, kb_lr_solve<NOISE_EXTERNAL>, C.z.p, M, C.sqrt_sigma.p, -1.0, extlr, C.u.p, C.m,
-             nk, Lv.nsplit);
+    lr_solve_launch(M, noise, extlr, tag, Lv.nsplit, LrCarry{});
   }
 
   // x[rows(R)] += R u with u from the last lr_dot_solve
@@ -792,7 +827,7 @@ struct Engine : EngineBase {
     NoiseKey nk = key(lr_next.tag);
 #define MGMC_LRSU(NOISE)                                                                                        \
   launch(grid, 256, 0, C.stream, kb_lr_solve_update<T, NOISE>, xp, cs, R.row.p, R.ptr.p, R.col.p, R.val.p,      \
-         R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk, lr_next.nsplit)
+         R.nrows, C.z.p, lr_next.M, C.sqrt_sigma.p, -1.0, lr_next.extlr, C.m, C.K, nk, lr_next.nsplit, lr_next.carry)
     if (lr_next.noise == NOISE_NONE) MGMC_LRSU(NOISE_NONE);
     else if (lr_next.noise == NOISE_PHILOX) MGMC_LRSU(NOISE_PHILOX);
     else MGMC_LRSU(NOISE_EXTERNAL);
@@ -892,6 +927,9 @@ struct Engine : EngineBase {
       int d = forward ? 1 : 0;
       lr_dot_solve(l, X(l), Lv.Sinv[d].p, noise, extlr, tag);
       lr_apply(X(l), Lv.g.cs, Lv.W[d]);
+      carry_l = l;
+      carry_dir = d;
+      lr_carry = lr_next;
     }
   }
 
@@ -994,7 +1032,7 @@ struct Engine : EngineBase {
     if (rr_ok(l)) {
       residual_restrict_fused(l);
       if (C.m > 0) {
-        lr_dot_solve(l, X(l), C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+        lr_residual_solve(l, X(l));
         lr_apply(f[l + 1].p, Cc.g.cs, Cc.Brows);
       }
       return;
@@ -1013,7 +1051,7 @@ struct Engine : EngineBase {
       launch(gq, 256, 0, C.stream, kb_restrict<T, 2>, r[l].p, f[l + 1].p, F.g, Cc.g);
     }
     if (C.m > 0) {
-      lr_dot_solve(l, X(l), C.sigma_inv_mat.p, NOISE_NONE, nullptr, 0);
+      lr_residual_solve(l, X(l));
       lr_apply(f[l + 1].p, Cc.g.cs, Cc.Brows);
     }
   }
@@ -1021,6 +1059,7 @@ struct Engine : EngineBase {
   // x_l += alpha P x_{l+1}; folded into the next sweep when that is a fused
   // sweep of level l (the post-smoother follows the prolongation directly)
   void prolongate_or_defer(int l, double alpha, bool sweep_follows) {
+    end_carry(l);
     if (sweep_follows && C.tune_fold_prolong && fused_ok(l)) {
       pending[l] = 1;
       pending_alpha[l] = alpha;
@@ -1030,6 +1069,7 @@ struct Engine : EngineBase {
   }
 
   void prolongate(int l, double alpha) override {
+    end_carry(l);
     Level& F = *C.L[l];
     Level& Cc = *C.L[l + 1];
     dim3 gr = row_grid(F.g, F.g.px >> 3);
@@ -1047,6 +1087,7 @@ struct Engine : EngineBase {
     int l = C.nlevel - 1;
     Level& Lv = *C.L[l];
     xzero[l] = 0;  // k_coarse writes every interior value and reads no x
+    end_carry(l);
     uint32_t tag = mode == NOISE_NONE ? 0 : C.tag++;
     NoiseKey nk = key(tag);
     const long long fcs = geo(l).fcs;
@@ -1095,7 +1136,10 @@ struct Engine : EngineBase {
 #undef MGMC_COARSE
   }
 
-  void zero_x(int l) { fill(cur[l] ? x2[l].p : x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0); }
+  void zero_x(int l) {
+    end_carry(l);
+    fill(cur[l] ? x2[l].p : x[l].p, (long long)C.L[l]->g.cs * C.K, (T)0);
+  }
   // x_l = 0 at the start of a coarse-level visit. Boundary slots, padding and
   // guards are zero from allocation and no kernel writes them with anything
   // else, so a consumer that writes every interior value without reading x
@@ -1103,6 +1147,7 @@ struct Engine : EngineBase {
   // the coarsest level) and the plane-parity sweep, which then skips its x
   // loads (first launch: every plane zero; second launch: its own planes).
   void zero_x_lazy(int l) {
+    end_carry(l);
     if (l == C.nlevel - 1 && C.p.coarse_solver == MGMC_COARSE_CHOLESKY && C.Nc > 0) {
       xzero[l] = 1;
       return;
@@ -1171,12 +1216,14 @@ struct Engine : EngineBase {
 
   void cycle_sample() override {
     C.tag = 1;
+    carry_l = -1;
     sample(0);
     rec_mark(2, 0, 0);
     normalise();
   }
   void cycle_precondition() override {
     C.tag = 1;
+    carry_l = -1;
     solve(0);
     normalise();
   }
@@ -1184,6 +1231,7 @@ struct Engine : EngineBase {
   void pack(int l, int which, const double* ref_dev, int per_chain) override {
     Level& Lv = *C.L[l];
     T* dst = which == 0 ? X(l) : which == 1 ? f[l].p : ext.p;
+    if (which == 0) end_carry(l);
     launch(nblocks(Lv.N * C.K), 256, 0, C.stream, k_pack<T>, ref_dev, dst, Lv.g, C.K, per_chain);
     if (l == 0 && which == 1) f0_shared = per_chain == 0;
   }
@@ -1285,7 +1333,7 @@ struct Engine : EngineBase {
       launch(grid, 256, 0, C.stream, k_lr_dot<double, 1>, x64, F.g.cs, F.bptr.p, F.bidx.p, F.bval.p, C.m, C.K, C.z.p,
              F.nsplit);
       launch(C.K, MMAX, 0, C.stream, k_lr_solve<NOISE_NONE>, C.z.p, C.sigma_inv_mat.p, C.sqrt_sigma.p, -1.0, nullptr,
-             C.u.p, C.m, nk, F.nsplit);
+             C.u.p, C.m, nk, F.nsplit, LrCarry{});
       dim3 g2(nblocks(F.Brows.nrows), (C.K + LR_CPU - 1) / LR_CPU);
       if (F.Brows.nrows)
         launch(g2, 256, 0, C.stream, k_lr_update<double, LR_CPU>, r64, F.g.cs, F.Brows.row.p, F.Brows.ptr.p, F.Brows.col.p,
@@ -1432,6 +1480,7 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
       for (int d = 0; d < 2; ++d) {
         Lv->W[d].upload(HL.W[d]);
         Lv->Sinv[d].upload(HL.Sinv[d].data(), HL.Sinv[d].size());
+        Lv->G[d].upload(HL.G[d].data(), HL.G[d].size());
       }
     }
     C.L.push_back(std::move(Lv));
@@ -2423,6 +2472,8 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
     h->C.tune_rb_chains = value;
   } else if (k == "rb_balanced") {
     h->C.tune_rb_balanced = value != 0;
+  } else if (k == "lr_carry") {
+    h->C.tune_lr_carry = value != 0;
   } else
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
@@ -2442,6 +2493,7 @@ static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_blocks") return h->C.tune_rb_blocks;
   if (k == "rb_chains") return h->C.tune_rb_chains;
   if (k == "rb_balanced") return h->C.tune_rb_balanced;
+  if (k == "lr_carry") return h->C.tune_lr_carry;
   if (k == "lr_dot_chains") return h->C.tune_lr_dot_chains;
   if (k == "rb_min_planes") return h->C.tune_rb_min_planes;
   if (k == "rr_min_planes") return h->C.tune_rr_min_planes;

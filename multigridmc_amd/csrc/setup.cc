@@ -573,7 +573,7 @@ static void compute_lowrank(HostLevel& L, const std::vector<Column>& cols, const
       }
       W.ptr.push_back((int32_t)W.col.size());
     }
-    // S = Sigma + B^T W
+    // G = B^T W, S = Sigma + G
     std::vector<double> S((size_t)m * m, 0.0);
     for (int k = 0; k < m; ++k) {
       for (auto& e : cols[k]) {
@@ -582,8 +582,9 @@ static void compute_lowrank(HostLevel& L, const std::vector<Column>& cols, const
         int r = it->second;
         for (int a = W.ptr[r]; a < W.ptr[r + 1]; ++a) S[(size_t)k * m + W.col[a]] += e.second * W.val[a];
       }
-      S[(size_t)k * m + k] += sigma[k];
     }
+    L.G[dir] = S;
+    for (int k = 0; k < m; ++k) S[(size_t)k * m + k] += sigma[k];
     L.Sinv[dir] = inverse(S, m);
   }
 }

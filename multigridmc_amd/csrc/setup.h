@@ -87,6 +87,7 @@ struct HostLevel {
   SparseRows Brows;      // B_l by rows (coarse-side term of the residual)
   SparseRows W[2];       // W = (L_colour + D/omega)^{-1} B, [0] backward, [1] forward
   std::vector<double> Sinv[2];  // (Sigma + B^T W)^{-1}, m x m row-major
+  std::vector<double> G[2];     // B^T W, m x m row-major
 };
 
 struct HostCoarse {

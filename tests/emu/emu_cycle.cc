@@ -273,7 +273,9 @@ void batch_case() {
 // evenly over the resident workgroups (rb_balanced: runs that cross from one
 // column into the next) must give the samples of its z-segment form, and
 // with the coarse correction added while the first post-sweep reads its
-// planes, those of a separate prolongation launch (to rounding).
+// planes, those of a separate prolongation launch (to rounding); the
+// residuals' low-rank terms carried from the SMW steps before them, those of
+// a second gather over x (to rounding).
 void balanced_case() {
   std::printf("case balanced level-0 sweep split\n");
   std::fflush(stdout);
@@ -284,16 +286,17 @@ void balanced_case() {
     ProblemData d;
     build(c, d);
     const int64_t N = d.N;
-    std::vector<double> f((size_t)N), x((size_t)(N * c.K), 0.0), st[3], obs[3];
+    std::vector<double> f((size_t)N), x((size_t)(N * c.K), 0.0), st[4], obs[4];
     for (int64_t v = 0; v < N; ++v) f[(size_t)v] = std::sin(0.21 * (double)v);
     const int64_t probes[2] = {3, N / 2};
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 4; ++v) {
       const int bal = v == 0 ? 0 : 1;
       mgmc_handle* h = nullptr;
       check(mgmc_create(&d.p, &d.par, &h), "mgmc_create");
       check(mgmc_set_tuning(h, "rb_threads", 64), "mgmc_set_tuning");
       check(mgmc_set_tuning(h, "rb_balanced", bal), "mgmc_set_tuning");
       if (v == 2) check(mgmc_set_tuning(h, "fold_prolongation", 0), "mgmc_set_tuning");
+      if (v == 3) check(mgmc_set_tuning(h, "lr_carry", 0), "mgmc_set_tuning");
       int got = -1;
       check(mgmc_get_tuning(h, "rb_balanced", &got), "mgmc_get_tuning");
       if (got != bal) {
@@ -314,16 +317,18 @@ void balanced_case() {
       std::printf("FAIL balanced: %s samples differ from the z-segment split\n", precision == MGMC_FP32 ? "fp32" : "fp64");
       ++failures;
     }
-    double dmax = 0.0, amax = 0.0;
-    for (size_t i = 0; i < st[1].size(); ++i) {
-      dmax = std::max(dmax, std::fabs(st[1][i] - st[2][i]));
-      amax = std::max(amax, std::fabs(st[2][i]));
-    }
     const double tol = precision == MGMC_FP64 ? 1e-12 : 2e-5;
-    if (!(dmax <= tol * amax)) {
-      std::printf("FAIL fold: %s folded coarse correction differs from the separate launch by %.3e (max |x| %.3e)\n",
-                  precision == MGMC_FP32 ? "fp32" : "fp64", dmax, amax);
-      ++failures;
+    for (int v = 2; v < 4; ++v) {
+      double dmax = 0.0, amax = 0.0;
+      for (size_t i = 0; i < st[1].size(); ++i) {
+        dmax = std::max(dmax, std::fabs(st[1][i] - st[v][i]));
+        amax = std::max(amax, std::fabs(st[v][i]));
+      }
+      if (!(dmax <= tol * amax)) {
+        std::printf("FAIL %s: %s differs from the reference form by %.3e (max |x| %.3e)\n",
+                    v == 2 ? "fold" : "carry", precision == MGMC_FP32 ? "fp32" : "fp64", dmax, amax);
+        ++failures;
+      }
     }
   }
 }
@@ -334,6 +339,8 @@ int main() {
   const Case cases[] = {
       {"3d 16^3, 3 measurements, fp64, multicolour", 3, {16, 16, 16}, 3, 3, MGMC_FP64, MGMC_ORDER_MULTICOLOUR, 2, false},
       {"3d 16^3, 3 measurements, fp32, multicolour", 3, {16, 16, 16}, 3, 3, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 2, false},
+      // more chains than one low-rank update workgroup serves: separate solve launch
+      {"3d 8^3, 2 measurements, fp32, 9 chains", 3, {8, 8, 8}, 2, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 9, false},
       {"3d 12x16x8 (x not a multiple of 8), fp64", 3, {12, 16, 8}, 2, 2, MGMC_FP64, MGMC_ORDER_MULTICOLOUR, 1, false},
       {"2d 24x20, fp32", 2, {24, 20, 1}, 3, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 2, false},
       {"3d 8^3, lexicographic, fp64", 3, {8, 8, 8}, 2, 2, MGMC_FP64, MGMC_ORDER_LEXICOGRAPHIC, 1, false},
