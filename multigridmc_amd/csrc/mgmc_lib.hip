@@ -655,13 +655,16 @@ struct Engine : EngineBase {
 
   void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, kb_fill<T>, p, n, v); }
 
-  // HIP events around level-0 sweep launches when profiling
+  // HIP events around level-0 sweep launches when profiling. They only time:
+  // without the system-scope fence a default event record ends with (an L2
+  // writeback), which the launch itself does not pay; mgmc_profile_result
+  // synchronises the stream before reading them.
   bool prof_begin(int l) {
     if (!(C.prof && l == 0) || g_rec) return false;
     if (C.ev_used + 2 > C.ev.size()) {
       for (int q = 0; q < 64; ++q) {
         hipEvent_t e;
-        HIP_CHECK(hipEventCreate(&e));
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         C.ev.push_back(e);
       }
     }

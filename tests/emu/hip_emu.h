@@ -144,7 +144,7 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) {
   return hipSuccess;
 }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
-enum { hipEventDisableTiming = 2 };
+enum { hipEventDisableTiming = 2, hipEventDisableSystemFence = 0x20000000 };
 inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
