@@ -1562,9 +1562,10 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
     a = (b / nseg * ntile + (int)bk.x % ntile) * P + (b % nseg) * SZ;
     e = a + SZ < (a / P + 1) * P ? a + SZ : (a / P + 1) * P;
   } else {
-    const long long total = (long long)ncol * P;
-    a = (int)(total * bk.x / bk.gx);
-    e = (int)(total * (bk.x + 1) / bk.gx);
+    // equal runs: the first total % grid workgroups take one unit more
+    const int total = ncol * P, q = total / (int)bk.gx, rem = total % (int)bk.gx, w = (int)bk.x;
+    a = w * q + (w < rem ? w : rem);
+    e = a + q + (w < rem ? 1 : 0);
   }
   a = __builtin_amdgcn_readfirstlane(a);
   e = __builtin_amdgcn_readfirstlane(e);
@@ -2005,9 +2006,10 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
     a = (b / nseg * ntile + (int)bk.x % ntile) * P + (b % nseg) * SZc;
     e = a + SZc < (a / P + 1) * P ? a + SZc : (a / P + 1) * P;
   } else {
-    const long long total = (long long)ncol * P;
-    a = (int)(total * bk.x / bk.gx);
-    e = (int)(total * (bk.x + 1) / bk.gx);
+    // equal runs: the first total % grid workgroups take one unit more
+    const int total = ncol * P, q = total / (int)bk.gx, rem = total % (int)bk.gx, w = (int)bk.x;
+    a = w * q + (w < rem ? w : rem);
+    e = a + q + (w < rem ? 1 : 0);
   }
   a = __builtin_amdgcn_readfirstlane(a);
   e = __builtin_amdgcn_readfirstlane(e);
