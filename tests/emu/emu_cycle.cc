@@ -139,11 +139,11 @@ void run(const Case& c) {
 }
 
 // A batch of 3D and 2D instances of different sizes, precisions and chain
-// counts:
+// counts (one with more chains than a low-rank update workgroup serves):
 // the merged launches (mgmc_batch_run_dev) must give every instance the
 // observables and final state of its own run, bit for bit.
 void batch_case() {
-  std::printf("case batch of 6 instances, 3d and 2d (merged launches)\n");
+  std::printf("case batch of 7 instances, 3d and 2d (merged launches)\n");
   std::fflush(stdout);
   const Case cs[] = {
       {"16^3 fp32", 3, {16, 16, 16}, 3, 3, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 2, false},
@@ -152,8 +152,10 @@ void batch_case() {
       {"16^3 fp32 no measurements", 3, {16, 16, 16}, 0, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 1, false},
       {"2d 24x20 fp32", 2, {24, 20, 1}, 3, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 2, false},
       {"2d 32x32 fp64", 2, {32, 32, 1}, 2, 3, MGMC_FP64, MGMC_ORDER_MULTICOLOUR, 1, false},
+      // more chains than one low-rank update workgroup serves: separate solve launches
+      {"8^3 fp32, 9 chains", 3, {8, 8, 8}, 2, 2, MGMC_FP32, MGMC_ORDER_MULTICOLOUR, 9, false},
   };
-  const int n = 6, ncycles = 2;
+  const int n = 7, ncycles = 2;
   ProblemData d[n];
   mgmc_handle* own[n];
   mgmc_handle* bat[n];

@@ -77,6 +77,7 @@ def test_batch_forms_agree():
     inst[2]["nchains"] = 1
     inst[4]["precision"] = "fp64"
     inst[4]["nchains"] = 5
+    inst[5]["nchains"] = 12  # separate low-rank solve launches (more than 8 chains)
     ref = [_sequential(i, 3, 20) for i in inst]
     for mode, persistent, graph in (("merged", 0, True), ("merged", 0, False), ("merged", 1, True),
                                     ("merged", 2, True), ("streams", 0, True)):

@@ -41,8 +41,8 @@ def test_kernels_under_emulation_asan_and_ubsan(tmp_path):
     per GPU thread, malloc'd device memory and LDS) under AddressSanitizer and
     UndefinedBehaviorSanitizer: sampling cycles and multigrid solves in FP64 and
     FP32, multicolour and lexicographic order, constant and variable stencils,
-    x extents that are and are not multiples of 8, and a batch of six 3D and 2D
-    instances of different sizes whose merged launches (plain and persistent)
+    x extents that are and are not multiples of 8, and a batch of seven 3D and 2D
+    instances of different sizes and chain counts whose merged launches (plain and persistent)
     must reproduce every instance's own run bit for bit (tests/emu/emu_cycle.cc).
     Any out-of-bounds device or LDS access aborts the run."""
     exe = str(tmp_path / "emu_cycle")
