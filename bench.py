@@ -110,6 +110,22 @@ def timed_run(s, warmup, steps, rows, use_graph):
     return t, obs[:steps].cpu().numpy()
 
 
+def repeat_runs(s, steps, n, use_graph):
+    """`n` more timed runs of `steps` cycles each (one device buffer; one
+    untimed cycle first captures the graph for it): ms per cycle of each."""
+    import torch
+    obs = torch.empty((max(steps, 1), s.nchains, s.nprobe), dtype=torch.float64, device="cuda")
+    s.run_dev(1, obs.data_ptr(), use_graph)
+    out = []
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run_dev(steps, obs.data_ptr(), use_graph)
+        torch.cuda.synchronize()
+        out.append(1e3 * (time.perf_counter() - t0) / steps)
+    return out
+
+
 def host_cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -282,6 +298,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="further timed runs of each form after the measurement (median reported)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-batch timing in the metric line")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--colour-phases", action="store_true",
@@ -403,6 +421,19 @@ def main():
     tau64, tau64_med = tau_max(obs64)
     ess64 = args.fp64_steps / (t64 * tau64)
 
+    # repeats: --repeats more timed runs of each form (FP64: at most 2,000
+    # cycles each), reported as their median beside the single timed run
+    repeats = None
+    if args.repeats > 0:
+        r32 = repeat_runs(s32, args.steps, args.repeats, g)
+        n64 = min(args.fp64_steps, 2000)
+        r64 = repeat_runs(s64, n64, args.repeats, g)
+        m32, m64 = float(np.median(r32)), float(np.median(r64))
+        repeats = {"n": args.repeats, "fp32_steps_each": args.steps, "fp32_ms_per_step": r32,
+                   "fp32_ms_per_step_median": m32, "fp32_value_at_median": 1e3 * K / (m32 * tau32),
+                   "fp64_steps_each": n64, "fp64_ms_per_step": r64, "fp64_ms_per_step_median": m64,
+                   "fp64_value_at_median": 1e3 / (m64 * tau64)}
+
     # exact probe moments from FP64 multigrid solves (the posterior mean and
     # b_p^T Q^{-1} b_q), against which both forms' samples are checked
     cov_exact = None
@@ -443,6 +474,7 @@ def main():
         "fp64_single_chain": {"value": ess64, "unit": "indep. samples/s", "steps": args.fp64_steps,
                               "ms_per_step": 1e3 * t64 / args.fp64_steps},
         "speedup_vs_fp64_single": ess32 / ess64,
+        "repeats": repeats,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel,
