@@ -719,12 +719,14 @@ __device__ __forceinline__ void b_lr_solve(const Blk bk, const double* __restric
     __syncthreads();
     if (tid < m) {
       double acc = 0.0;
+#pragma unroll 8
       for (int l = 0; l < m; ++l) acc += cy.Mp[tid * m + l] * a[l];
       t[tid] = acc;
     }
     __syncthreads();
     if (tid < m) {
       double acc = zs;
+#pragma unroll 8
       for (int l = 0; l < m; ++l) acc += cy.G[tid * m + l] * t[l];
       zs = acc;
     }
@@ -743,6 +745,7 @@ __device__ __forceinline__ void b_lr_solve(const Blk bk, const double* __restric
   __syncthreads();
   if (tid < m) {
     double acc = 0.0;
+#pragma unroll 8
     for (int l = 0; l < m; ++l) acc += M[tid * m + l] * a[l];
     u[(long long)c * m + tid] = acc;
   }
@@ -853,6 +856,7 @@ __device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ 
     for (int e = tid; e < nc * m; e += blockDim.x) {
       const int q = e / m, i = e % m;
       double acc = 0.0;
+#pragma unroll 8
       for (int l = 0; l < m; ++l) acc += cy.Mp[i * m + l] * a[q][l];
       us[q][i] = acc;
     }
@@ -864,6 +868,7 @@ __device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ 
     for (int p = 0; p < nsplit; ++p) zs += z[((long long)c * m + i) * nsplit + p];
     if (cy.G) {
       double acc = zs;
+#pragma unroll 8
       for (int l = 0; l < m; ++l) acc += cy.G[i * m + l] * us[q][l];
       zs = acc;
     }
@@ -881,6 +886,7 @@ __device__ __forceinline__ void b_lr_solve_update(const Blk bk, T* __restrict__ 
     const int q = e / m, i = e % m;
     const double* Mi = M + i * m;
     double acc = 0.0;
+#pragma unroll 8
     for (int l = 0; l < m; ++l) acc += Mi[l] * a[q][l];
     us[q][i] = acc;
   }
