@@ -330,7 +330,10 @@ int mgmc_level_stencil(const mgmc_handle* h, int level, double* stencil, int* nc
  * 8, 16: chains per thread of the low-rank update, with more than 8 chains),
  * "lr_carry" (0/1, default 1: the residual's B^T x taken from the SMW step
  * just before it, B^T (y + W u) = B^T y + (B^T W) u, instead of a second
- * gather over x). Results do not depend on these settings beyond rounding. */
+ * gather over x), "lr_split" (>= 64; entries of B per workgroup of the
+ * low-rank dot, whose columns are split into up to 8 parts: default 128 below
+ * 8 chains, else 512). Results do not depend on these settings beyond
+ * rounding. */
 int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
 
 /* Autotuner (no reference counterpart): coordinate search over the tuning
@@ -342,7 +345,7 @@ int mgmc_set_tuning(mgmc_handle* h, const char* key, int value);
  * so the sample path after tuning equals the untuned one bit for bit.
  * best_ms (may be NULL) receives the best time per cycle. */
 int mgmc_autotune(mgmc_handle* h, int ncycles, double* best_ms);
-/* Current value of one of the autotuned keys, or of "lr_carry". */
+/* Current value of one of the autotuned keys, or of "lr_carry" / "lr_split". */
 int mgmc_get_tuning(const mgmc_handle* h, const char* key, int* value);
 
 /* Profiling aid: with enable = 1 every level-0 sweep launch (one fused

@@ -154,6 +154,7 @@ struct Level {
   // low rank
   DevBuf<int32_t> bptr;
   int nsplit = 1;  // blocks per column of B in k_lr_dot
+  long long bcol = 0;  // entries per column of B (mean)
   DevBuf<long long> bidx;
   DevBuf<double> bval;
   DevRows Brows;
@@ -169,9 +170,8 @@ struct Level {
 };
 
 // k_lr_dot: B columns are split into up to kMaxLrSplit blocks of about
-// kLrSplitEntries entries (one batched gather trip of 256 threads each)
+// tune_lr_split entries (default 512: one batched gather trip of 256 threads)
 constexpr int kMaxLrSplit = 8;
-constexpr long long kLrSplitEntries = 512;
 
 struct Common {
   int K = 1;
@@ -222,6 +222,7 @@ struct Common {
   // the residual's B^T x from the SMW step just before it (z_p + B^T W u_p)
   // instead of a second gather over x
   int tune_lr_carry = 1;
+  int tune_lr_split = 512;  // entries of B per k_lr_dot workgroup (columns are split into up to 8 parts; set per K at creation)
   int tune_march = 0;         // single-pass 8-colour sweeps on 3D box levels
   int tune_march_blocks = 512; // ... workgroups they aim for
   int tune_march_min_planes = 4; // ... fewest planes per z-segment
@@ -500,6 +501,10 @@ struct Engine : EngineBase {
     cur[l] ^= 1;
   }
   void retune() override {
+    for (int l = 0; l < C.nlevel; ++l) {
+      Level& Lv = *C.L[l];
+      Lv.nsplit = (int)std::max(1LL, std::min((long long)kMaxLrSplit, Lv.bcol / C.tune_lr_split));
+    }
     for (int l = 0; l < C.nlevel; ++l)
       if ((fused_ok(l) || march_ok(l)) && !x2[l].p) x2[l].alloc((size_t)C.L[l]->g.cs * C.K, C.stream);
     HIP_CHECK(hipStreamSynchronize(C.stream));
@@ -1428,6 +1433,9 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
   Common& C = h->C;
   C.p = *par;
   C.K = par->nchains;
+  // fewer chains leave more room for parts of B's columns (one FP64 chain:
+  // 0.556 -> 0.552 ms per cycle; 64 FP32 chains: no change from 512 up)
+  C.tune_lr_split = C.K < 8 ? 128 : 512;
   C.m = prob->m;
   C.nlevel = par->nlevel;
   C.k0 = (uint32_t)par->seed;
@@ -1473,8 +1481,8 @@ int mgmc_create(const mgmc_problem* prob, const mgmc_params* par, mgmc_handle** 
       Lv->bptr.upload(HL.bptr.data(), HL.bptr.size());
       {
         const long long nnz = HL.bptr.empty() ? 0 : HL.bptr.back();
-        const long long per_col = C.m > 0 ? nnz / C.m : 0;
-        Lv->nsplit = (int)std::max(1LL, std::min((long long)kMaxLrSplit, per_col / kLrSplitEntries));
+        Lv->bcol = C.m > 0 ? nnz / C.m : 0;
+        Lv->nsplit = (int)std::max(1LL, std::min((long long)kMaxLrSplit, Lv->bcol / C.tune_lr_split));
       }
       std::vector<long long> bi(HL.bidx.begin(), HL.bidx.end());
       Lv->bidx.upload(bi.data(), bi.size());
@@ -2477,6 +2485,9 @@ static void apply_tuning(mgmc_handle* h, const std::string& k, int value) {
     h->C.tune_rb_balanced = value != 0;
   } else if (k == "lr_carry") {
     h->C.tune_lr_carry = value != 0;
+  } else if (k == "lr_split") {
+    if (value < 64) throw Error(MGMC_ERR_ARG, "lr_split must be >= 64");
+    h->C.tune_lr_split = value;
   } else
     throw Error(MGMC_ERR_ARG, "unknown tuning key '" + k + "'");
   h->E->retune();
@@ -2497,6 +2508,7 @@ static int get_tuning(const mgmc_handle* h, const std::string& k) {
   if (k == "rb_chains") return h->C.tune_rb_chains;
   if (k == "rb_balanced") return h->C.tune_rb_balanced;
   if (k == "lr_carry") return h->C.tune_lr_carry;
+  if (k == "lr_split") return h->C.tune_lr_split;
   if (k == "lr_dot_chains") return h->C.tune_lr_dot_chains;
   if (k == "rb_min_planes") return h->C.tune_rb_min_planes;
   if (k == "rr_min_planes") return h->C.tune_rr_min_planes;
