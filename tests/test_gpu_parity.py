@@ -270,6 +270,34 @@ def test_balanced_sweep_split_gives_the_same_samples(precision, n, K, threads):
     assert np.array_equal(states[0], states[1])
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("K", [3, 12])
+def test_carried_residual_term_and_dot_split_match_the_gather(precision, tol, K):
+    """The residual's low-rank term taken from the SMW step before it (B^T x =
+    z_p + B^T W u_p, lr_carry) and other splits of B's columns over the dot's
+    workgroups (lr_split) give the samples of the direct gather over x, to
+    rounding; with 3 chains the solves run inside the update launches, with
+    12 in launches of their own. Whole V- and W-cycles of a 3D posterior."""
+    op, _ = make_problem(3, (32, 24, 32), 4)
+    N = op.lattice.Nvertex
+    f = np.random.default_rng(14).standard_normal(N)
+    x0 = np.random.default_rng(15).standard_normal((K, N))
+    for cycle in (1, 2):
+        p = params(4, cycle=cycle)
+        states = []
+        for carry, split in ((0, 512), (1, 512), (1, 64), (1, 4096)):
+            s = mg.MultigridMCSampler(op, p, nchains=K, precision=precision, seed=31)
+            s.set_tuning("lr_carry", carry)
+            s.set_tuning("lr_split", split)
+            assert s.get_tuning("lr_carry") == carry and s.get_tuning("lr_split") == split
+            s.fix_rhs(f)
+            s.set_state(x0)
+            s.run(3, record=False)
+            states.append(s.get_state())
+        for k in range(1, len(states)):
+            assert np.abs(states[k] - states[0]).max() <= tol * np.abs(states[0]).max(), (cycle, k)
+
+
 @pytest.mark.parametrize("n,K", [((32, 16, 24), 4), ((24, 20, 16), 6)])
 @pytest.mark.parametrize("threads,blocks", [(512, 1024), (64, 1), (64, 100000)])
 def test_chain_blocked_sweep_matches_one_chain_per_workgroup(n, K, threads, blocks):
