@@ -1924,13 +1924,14 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
   const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
   const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
 
-  T xm[8], xc[8], xp[8], fb[8], fn[8];
+  T xm[8], xc[8], xp[8], xq[8], fb[8], fn[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = fb[q] = fn[q] = (T)0;
-  const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;
+  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = xq[q] = fb[q] = fn[q] = (T)0;
+  const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;  // p1 + 1 <= n2
   if (has_mem) {
     load8<T>(x + cbase + (long long)(p0 - 1) * pl, xm);
     load8<T>(x + cbase + (long long)p0 * pl, xc);
+    load8<T>(x + cbase + (long long)(p0 + 1) * pl, xp);
   }
   if (rvalid) load8<T>(f + fbase + (long long)p0 * pl, fb);
   T acc[4] = {(T)0, (T)0, (T)0, (T)0};
@@ -1939,8 +1940,9 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
   T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
 
   for (int p = p0; p <= p1; ++p) {
-    // planes p + 1 of x and f are in flight while plane p is processed
-    if (has_mem) load8<T>(x + cbase + (long long)(p + 1) * pl, xp);
+    // plane p + 2 of x and plane p + 1 of f are in flight while plane p is
+    // processed (x two planes ahead: plane p + 1 is read in this step)
+    if (has_mem && p + 2 <= g.n2) load8<T>(x + cbase + (long long)(p + 2) * pl, xq);
     if (rvalid && p < p1) load8<T>(f + fbase + (long long)(p + 1) * pl, fn);
     if (active) LdsRow<T>::store(LXrow, gx, ngx, xc);
     __syncthreads();
@@ -1999,6 +2001,7 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
     for (int q = 0; q < 8; ++q) {
       xm[q] = xc[q];
       xc[q] = xp[q];
+      xp[q] = xq[q];
       fb[q] = fn[q];
     }
   }
