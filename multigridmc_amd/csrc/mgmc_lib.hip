@@ -660,10 +660,10 @@ struct Engine : EngineBase {
 
   void fill(T* p, long long n, T v) { launch(nblocks(n), 256, 0, C.stream, kb_fill<T>, p, n, v); }
 
-  // HIP events around level-0 sweep launches when profiling. They only time:
-  // without the system-scope fence a default event record ends with (an L2
-  // writeback), which the launch itself does not pay; mgmc_profile_result
-  // synchronises the stream before reading them.
+  // HIP events around level-0 sweep launches when profiling. They only time,
+  // so they are created without the system-scope fence a default event record
+  // ends with (an L2 writeback that the launch itself does not pay);
+  // mgmc_profile_result synchronises the stream before reading them.
   bool prof_begin(int l) {
     if (!(C.prof && l == 0) || g_rec) return false;
     if (C.ev_used + 2 > C.ev.size()) {
@@ -770,7 +770,7 @@ struct Engine : EngineBase {
   }
   int carry_dir = 0;
   void end_carry(int l) {
-    if (carry_l == l || l < 0) carry_l = -1;
+    if (carry_l == l) carry_l = -1;
   }
   bool lr_fused() const { return C.K <= LR_CPU; }
   void lr_solve_launch(const double* M, int noise, const double* extlr, uint32_t tag, int nsplit, LrCarry cy) {
