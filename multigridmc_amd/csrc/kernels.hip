@@ -1418,140 +1418,140 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
 
   // the march of chain(s) `chain` over tile `tile`, owned interior planes [ks, ke)
   auto march = [&](const int tile, const int chain, const int ks, const int ke) {
-  const int j0 = 1 + tile * TY;
-  const int j = j0 - 2 + r;
-  const bool has_mem = active && j >= 0 && j <= g.n1;
-  const bool ring = active && j >= j0 - 1 && j <= j0 + TY && j >= 1 && j <= g.n1 - 1;
-  const bool owned = active && j >= j0 && j <= j0 + TY - 1 && j <= g.n1 - 1;
-  const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
-  const long long cbase1 = cbase + g.cs;  // second chain (CB = 2)
-  const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
+    const int j0 = 1 + tile * TY;
+    const int j = j0 - 2 + r;
+    const bool has_mem = active && j >= 0 && j <= g.n1;
+    const bool ring = active && j >= j0 - 1 && j <= j0 + TY && j >= 1 && j <= g.n1 - 1;
+    const bool owned = active && j >= j0 && j <= j0 + TY - 1 && j <= g.n1 - 1;
+    const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+    const long long cbase1 = cbase + g.cs;  // second chain (CB = 2)
+    const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
 
-  T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
-  T xmm1[8], xm1[8], xc1[8], xp1[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = xmm1[q] = xm1[q] = xc1[q] = xp1[q] = (T)0;
-  // optional prolongate-add of the coarse correction while the planes are read
-  ProlongStream<T> pro, pro1;
-#define MGMC_RB3_INIT(XM, XC, PRO, CH, CBASE)                                                                    \
-  PRO.init(xcoarse && has_mem ? xcoarse + (long long)(CH) * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1, \
-           alpha);                                                                                                 \
-  if (has_mem && ks - 2 >= 0) {                                                                                    \
-    load8<T>(x + CBASE + (ks - 2) * pl, XM);                                                                       \
-    PRO.add(XM, ks - 2);                                                                                           \
-  }                                                                                                                \
-  if (has_mem) {                                                                                                   \
-    load8<T>(x + CBASE + (ks - 1) * pl, XC);                                                                       \
-    PRO.add(XC, ks - 1);                                                                                           \
-  }
-  MGMC_RB3_INIT(xm, xc, pro, chain, cbase)
-  if constexpr (CB > 1) {
-    MGMC_RB3_INIT(xm1, xc1, pro1, chain + 1, cbase1)
-  }
-#undef MGMC_RB3_INIT
-
-  // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
-  // receives plane k+1; fm, fc = f on planes k-1, k
-  for (int k = ks - 1; k <= ke; ++k) {
-    const int kb = k - 1;
-    const bool doA = ring && inner && k >= 1 && k <= g.n2 - 1;
-    const bool storeB = owned && kb >= ks && kb <= ke - 1;
-    const bool doB = storeB && inner;
-    // loads first: next plane of x, f on plane k
-    // (the coarse rows of plane k+1's correction are requested with it and
-    // added after the noise, before the first colour reads plane k+1)
-    const bool next = has_mem && k + 1 <= g.n2;
-#define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
-  if (next) {                                             \
-    load8<T>(x + CBASE + (long long)(k + 1) * pl, XP);    \
-    PRO.advance(k + 1);                                   \
-  } else {                                                \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) XP[q] = (T)0; \
-  }
-    MGMC_RB3_LOAD(xp, pro, cbase)
-    if constexpr (CB > 1) {
-      MGMC_RB3_LOAD(xp1, pro1, cbase1)
+    T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
+    T xmm1[8], xm1[8], xc1[8], xp1[8];
+  #pragma unroll
+    for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = xmm1[q] = xm1[q] = xc1[q] = xp1[q] = (T)0;
+    // optional prolongate-add of the coarse correction while the planes are read
+    ProlongStream<T> pro, pro1;
+  #define MGMC_RB3_INIT(XM, XC, PRO, CH, CBASE)                                                                    \
+    PRO.init(xcoarse && has_mem ? xcoarse + (long long)(CH) * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1, \
+             alpha);                                                                                                 \
+    if (has_mem && ks - 2 >= 0) {                                                                                    \
+      load8<T>(x + CBASE + (ks - 2) * pl, XM);                                                                       \
+      PRO.add(XM, ks - 2);                                                                                           \
+    }                                                                                                                \
+    if (has_mem) {                                                                                                   \
+      load8<T>(x + CBASE + (ks - 1) * pl, XC);                                                                       \
+      PRO.add(XC, ks - 1);                                                                                           \
     }
-#undef MGMC_RB3_LOAD
-    if (doA) load8<T>(f + fbase + (long long)k * pl, fc);
-    // noise of both colours while the loads are in flight
-    const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
-    float xiA[4] = {0.f, 0.f, 0.f, 0.f}, xiB[4] = {0.f, 0.f, 0.f, 0.f};
-    float xiA1[4] = {0.f, 0.f, 0.f, 0.f}, xiB1[4] = {0.f, 0.f, 0.f, 0.f};
-    if (NOISE == NOISE_PHILOX) {
-      const uint32_t cj = (uint32_t)(j * ngx + gx);
-      const uint32_t cpl = (uint32_t)(g.py * ngx);
-      const uint32_t cA = ((k * cpl + cj) << 1) | (uint32_t)iparA, cB = ((kb * cpl + cj) << 1) | (uint32_t)iparB;
-      if (doA) normals4(cA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
-      if (doB) normals4(cB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+    MGMC_RB3_INIT(xm, xc, pro, chain, cbase)
+    if constexpr (CB > 1) {
+      MGMC_RB3_INIT(xm1, xc1, pro1, chain + 1, cbase1)
+    }
+  #undef MGMC_RB3_INIT
+
+    // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
+    // receives plane k+1; fm, fc = f on planes k-1, k
+    for (int k = ks - 1; k <= ke; ++k) {
+      const int kb = k - 1;
+      const bool doA = ring && inner && k >= 1 && k <= g.n2 - 1;
+      const bool storeB = owned && kb >= ks && kb <= ke - 1;
+      const bool doB = storeB && inner;
+      // loads first: next plane of x, f on plane k
+      // (the coarse rows of plane k+1's correction are requested with it and
+      // added after the noise, before the first colour reads plane k+1)
+      const bool next = has_mem && k + 1 <= g.n2;
+  #define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
+    if (next) {                                             \
+      load8<T>(x + CBASE + (long long)(k + 1) * pl, XP);    \
+      PRO.advance(k + 1);                                   \
+    } else {                                                \
+      _Pragma("unroll") for (int q = 0; q < 8; ++q) XP[q] = (T)0; \
+    }
+      MGMC_RB3_LOAD(xp, pro, cbase)
       if constexpr (CB > 1) {
-        if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiA1);
-        if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiB1);
+        MGMC_RB3_LOAD(xp1, pro1, cbase1)
       }
+  #undef MGMC_RB3_LOAD
+      if (doA) load8<T>(f + fbase + (long long)k * pl, fc);
+      // noise of both colours while the loads are in flight
+      const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
+      float xiA[4] = {0.f, 0.f, 0.f, 0.f}, xiB[4] = {0.f, 0.f, 0.f, 0.f};
+      float xiA1[4] = {0.f, 0.f, 0.f, 0.f}, xiB1[4] = {0.f, 0.f, 0.f, 0.f};
+      if (NOISE == NOISE_PHILOX) {
+        const uint32_t cj = (uint32_t)(j * ngx + gx);
+        const uint32_t cpl = (uint32_t)(g.py * ngx);
+        const uint32_t cA = ((k * cpl + cj) << 1) | (uint32_t)iparA, cB = ((kb * cpl + cj) << 1) | (uint32_t)iparB;
+        if (doA) normals4(cA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
+        if (doB) normals4(cB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+        if constexpr (CB > 1) {
+          if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiA1);
+          if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiB1);
+        }
+      }
+      if (next) {
+        pro.apply(xp, k + 1);
+        if constexpr (CB > 1) pro1.apply(xp1, k + 1);
+      }
+      if (active) LdsRow<T>::store(A0 + (k & 1) * asz + aoff, gx, ngx, xc);  // plane k before the first colour
+      if constexpr (CB > 1) {
+        if (active) LdsRow<T>::store(A0 + csz + (k & 1) * asz + aoff, gx, ngx, xc1);
+      }
+      __syncthreads();
+  #define MGMC_RB3_UPDATE(XMM, XM, XC, XP, XIA, XIB, CBASE, AC)                                                      \
+    {                                                                                                                \
+      const T* A = (AC) + (k & 1) * asz;                                                                             \
+      const T* Bm = (AC) + 2 * asz + (kb & 1) * bsz; /* plane k-1 after the first colour (written last step) */     \
+      T* Bk = (AC) + 2 * asz + (k & 1) * bsz;                                                                        \
+      /* first colour on plane k (owned rows and halo ring) */                                                       \
+      if (doA) {                                                                                                     \
+        const T* Arow = A + aoff;                                                                                    \
+        T ym[8], yp[8];                                                                                              \
+        LdsRow<T>::load(Arow - g.px, gx, ngx, ym);                                                                   \
+        LdsRow<T>::load(Arow + g.px, gx, ngx, yp);                                                                   \
+        const T left = gx > 0 ? LdsRow<T>::left(Arow, gx, ngx) : (T)0;                                               \
+        const T right = gx < ngx - 1 ? LdsRow<T>::right(Arow, gx, ngx) : (T)0;                                       \
+        const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)k * pl : nullptr;                           \
+        if (iparA == 0)                                                                                              \
+          star_update4<0, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
+                                    ns);                                                                             \
+        else                                                                                                         \
+          star_update4<1, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
+                                    ns);                                                                             \
+      }                                                                                                              \
+      /* second colour on plane k-1 (owned rows, owned planes) */                                                    \
+      if (doB) {                                                                                                     \
+        const T* Brow = Bm + boff;                                                                                   \
+        T ym[8], yp[8];                                                                                              \
+        LdsRow<T>::load(Brow - g.px, gx, ngx, ym);                                                                   \
+        LdsRow<T>::load(Brow + g.px, gx, ngx, yp);                                                                   \
+        const T left = gx > 0 ? LdsRow<T>::left(Brow, gx, ngx) : (T)0;                                               \
+        const T right = gx < ngx - 1 ? LdsRow<T>::right(Brow, gx, ngx) : (T)0;                                       \
+        const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)kb * pl : nullptr;                          \
+        if (iparB == 0)                                                                                              \
+          star_update4<0, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
+                                    om_d, ns);                                                                       \
+        else                                                                                                         \
+          star_update4<1, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
+                                    om_d, ns);                                                                       \
+      }                                                                                                              \
+      if (storeB) store8<T>(xout + CBASE + (long long)kb * pl, XM);                                                  \
+      /* plane k after the first colour, for the second colour at the next step */                                   \
+      if (active && r >= 1 && r <= TY + 2) LdsRow<T>::store(Bk + boff, gx, ngx, XC);                                 \
+      _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                                                \
+        XMM[q] = XM[q];                                                                                              \
+        XM[q] = XC[q];                                                                                               \
+        XC[q] = XP[q];                                                                                               \
+      }                                                                                                              \
     }
-    if (next) {
-      pro.apply(xp, k + 1);
-      if constexpr (CB > 1) pro1.apply(xp1, k + 1);
+      MGMC_RB3_UPDATE(xmm, xm, xc, xp, xiA, xiB, cbase, A0)
+      if constexpr (CB > 1) {
+        MGMC_RB3_UPDATE(xmm1, xm1, xc1, xp1, xiA1, xiB1, cbase1, A0 + csz)
+      }
+  #undef MGMC_RB3_UPDATE
+  #pragma unroll
+      for (int q = 0; q < 8; ++q) fm[q] = fc[q];
     }
-    if (active) LdsRow<T>::store(A0 + (k & 1) * asz + aoff, gx, ngx, xc);  // plane k before the first colour
-    if constexpr (CB > 1) {
-      if (active) LdsRow<T>::store(A0 + csz + (k & 1) * asz + aoff, gx, ngx, xc1);
-    }
-    __syncthreads();
-#define MGMC_RB3_UPDATE(XMM, XM, XC, XP, XIA, XIB, CBASE, AC)                                                      \
-  {                                                                                                                \
-    const T* A = (AC) + (k & 1) * asz;                                                                             \
-    const T* Bm = (AC) + 2 * asz + (kb & 1) * bsz; /* plane k-1 after the first colour (written last step) */     \
-    T* Bk = (AC) + 2 * asz + (k & 1) * bsz;                                                                        \
-    /* first colour on plane k (owned rows and halo ring) */                                                       \
-    if (doA) {                                                                                                     \
-      const T* Arow = A + aoff;                                                                                    \
-      T ym[8], yp[8];                                                                                              \
-      LdsRow<T>::load(Arow - g.px, gx, ngx, ym);                                                                   \
-      LdsRow<T>::load(Arow + g.px, gx, ngx, yp);                                                                   \
-      const T left = gx > 0 ? LdsRow<T>::left(Arow, gx, ngx) : (T)0;                                               \
-      const T right = gx < ngx - 1 ? LdsRow<T>::right(Arow, gx, ngx) : (T)0;                                       \
-      const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)k * pl : nullptr;                           \
-      if (iparA == 0)                                                                                              \
-        star_update4<0, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
-                                  ns);                                                                             \
-      else                                                                                                         \
-        star_update4<1, T, NOISE>(XC, left, right, ym, yp, XM, XP, fc, XIA, eg, 8 * gx, g.n0, cx, cy, cz, D, om_d, \
-                                  ns);                                                                             \
-    }                                                                                                              \
-    /* second colour on plane k-1 (owned rows, owned planes) */                                                    \
-    if (doB) {                                                                                                     \
-      const T* Brow = Bm + boff;                                                                                   \
-      T ym[8], yp[8];                                                                                              \
-      LdsRow<T>::load(Brow - g.px, gx, ngx, ym);                                                                   \
-      LdsRow<T>::load(Brow + g.px, gx, ngx, yp);                                                                   \
-      const T left = gx > 0 ? LdsRow<T>::left(Brow, gx, ngx) : (T)0;                                               \
-      const T right = gx < ngx - 1 ? LdsRow<T>::right(Brow, gx, ngx) : (T)0;                                       \
-      const T* eg = NOISE == NOISE_EXTERNAL ? ext + CBASE + (long long)kb * pl : nullptr;                          \
-      if (iparB == 0)                                                                                              \
-        star_update4<0, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
-                                  om_d, ns);                                                                       \
-      else                                                                                                         \
-        star_update4<1, T, NOISE>(XM, left, right, ym, yp, XMM, XC, fm, XIB, eg, 8 * gx, g.n0, cx, cy, cz, D,      \
-                                  om_d, ns);                                                                       \
-    }                                                                                                              \
-    if (storeB) store8<T>(xout + CBASE + (long long)kb * pl, XM);                                                  \
-    /* plane k after the first colour, for the second colour at the next step */                                   \
-    if (active && r >= 1 && r <= TY + 2) LdsRow<T>::store(Bk + boff, gx, ngx, XC);                                 \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                                                \
-      XMM[q] = XM[q];                                                                                              \
-      XM[q] = XC[q];                                                                                               \
-      XC[q] = XP[q];                                                                                               \
-    }                                                                                                              \
-  }
-    MGMC_RB3_UPDATE(xmm, xm, xc, xp, xiA, xiB, cbase, A0)
-    if constexpr (CB > 1) {
-      MGMC_RB3_UPDATE(xmm1, xm1, xc1, xp1, xiA1, xiB1, cbase1, A0 + csz)
-    }
-#undef MGMC_RB3_UPDATE
-#pragma unroll
-    for (int q = 0; q < 8; ++q) fm[q] = fc[q];
-  }
   };
 
   // nseg > 0: workgroup (chain group * nseg + seg) * ntile + tile marches the
@@ -1915,96 +1915,96 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
 
   // the march of chain `chain` over tile `tile`, coarse planes [Ks, Ke)
   auto march = [&](const int tile, const int chain, const int Ks, const int Ke) {
-  const int J0 = 1 + tile * TYc;
-  const int j = 2 * J0 - 2 + rt;
-  const bool has_mem = active && j >= 0 && j <= g.n1;
-  const bool rvalid = rrow && j >= 1 && j <= g.n1 - 1;
-  const int J = j >> 1;
-  const bool crow = rrow && !(j & 1) && J >= J0 && J < J0 + TYc && J <= gc.n1 - 1;
-  const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
-  const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
+    const int J0 = 1 + tile * TYc;
+    const int j = 2 * J0 - 2 + rt;
+    const bool has_mem = active && j >= 0 && j <= g.n1;
+    const bool rvalid = rrow && j >= 1 && j <= g.n1 - 1;
+    const int J = j >> 1;
+    const bool crow = rrow && !(j & 1) && J >= J0 && J < J0 + TYc && J <= gc.n1 - 1;
+    const long long cbase = (long long)chain * g.cs + (long long)j * g.px + 8 * gx;
+    const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
 
-  T xm[8], xc[8], xp[8], xq[8], fb[8], fn[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = xq[q] = fb[q] = fn[q] = (T)0;
-  const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;  // p1 + 1 <= n2
-  if (has_mem) {
-    load8<T>(x + cbase + (long long)(p0 - 1) * pl, xm);
-    load8<T>(x + cbase + (long long)p0 * pl, xc);
-    load8<T>(x + cbase + (long long)(p0 + 1) * pl, xp);
-  }
-  if (rvalid) load8<T>(f + fbase + (long long)p0 * pl, fb);
-  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
-  T* LXrow = LX + (size_t)rt * g.px;  // LdsRow chunks
-  T* LRrow = LR + (size_t)(rt - 1) * g.px;
-  T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
+    T xm[8], xc[8], xp[8], xq[8], fb[8], fn[8];
+  #pragma unroll
+    for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = xq[q] = fb[q] = fn[q] = (T)0;
+    const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;  // p1 + 1 <= n2
+    if (has_mem) {
+      load8<T>(x + cbase + (long long)(p0 - 1) * pl, xm);
+      load8<T>(x + cbase + (long long)p0 * pl, xc);
+      load8<T>(x + cbase + (long long)(p0 + 1) * pl, xp);
+    }
+    if (rvalid) load8<T>(f + fbase + (long long)p0 * pl, fb);
+    T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+    T* LXrow = LX + (size_t)rt * g.px;  // LdsRow chunks
+    T* LRrow = LR + (size_t)(rt - 1) * g.px;
+    T* LTrow = LT + (size_t)(rt - 1) * (g.px >> 1) + 4 * gx;
 
-  for (int p = p0; p <= p1; ++p) {
-    // plane p + 2 of x and plane p + 1 of f are in flight while plane p is
-    // processed (x two planes ahead: plane p + 1 is read in this step)
-    if (has_mem && p + 2 <= g.n2) load8<T>(x + cbase + (long long)(p + 2) * pl, xq);
-    if (rvalid && p < p1) load8<T>(f + fbase + (long long)(p + 1) * pl, fn);
-    if (active) LdsRow<T>::store(LXrow, gx, ngx, xc);
-    __syncthreads();
-    T r[8];
-    if (rrow) {
-      T ym[8], yp[8];
-      LdsRow<T>::load(LXrow - g.px, gx, ngx, ym);
-      LdsRow<T>::load(LXrow + g.px, gx, ngx, yp);
-      const T left = gx > 0 ? LdsRow<T>::left(LXrow, gx, ngx) : (T)0;
-      const T right = gx < ngx - 1 ? LdsRow<T>::right(LXrow, gx, ngx) : (T)0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const T xl = q > 0 ? xc[q - 1] : left;
-        const T xr = q < 7 ? xc[q + 1] : right;
-        T a = cy * ym[q] + cx * xl + D * xc[q] + cx * xr + cy * yp[q];
-        a = cz * xm[q] + a + cz * xp[q];
-        const int i = 8 * gx + q;
-        r[q] = (rvalid && i >= 1 && i <= g.n0 - 1) ? fb[q] - a : (T)0;
-      }
-      LdsRow<T>::store(LRrow, gx, ngx, r);
-    }
-    __syncthreads();
-    if (rrow) {
-      const T rl = gx > 0 ? LdsRow<T>::left(LRrow, gx, ngx) : (T)0;
-      T t[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) t[a] = half * (a > 0 ? r[2 * a - 1] : rl) + r[2 * a] + half * r[2 * a + 1];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) LTrow[a] = t[a];
-    }
-    __syncthreads();
-    if (crow) {
-      const int hp = g.px >> 1;
-      T u[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) u[a] = half * LTrow[a - hp] + LTrow[a] + half * LTrow[a + hp];
-      if (p & 1) {
-        // fine plane 2K+1 completes coarse plane K = (p-1)/2 and starts (p+1)/2
-        const int K = (p - 1) >> 1;
-        if (K >= Ks) {
-          T* out = fc + (long long)chain * gc.cs + ((long long)K * gc.py + J) * gc.px + 4 * gx;
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            const int I = 4 * gx + a;
-            if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a] + half * u[a];
-          }
+    for (int p = p0; p <= p1; ++p) {
+      // plane p + 2 of x and plane p + 1 of f are in flight while plane p is
+      // processed (x two planes ahead: plane p + 1 is read in this step)
+      if (has_mem && p + 2 <= g.n2) load8<T>(x + cbase + (long long)(p + 2) * pl, xq);
+      if (rvalid && p < p1) load8<T>(f + fbase + (long long)(p + 1) * pl, fn);
+      if (active) LdsRow<T>::store(LXrow, gx, ngx, xc);
+      __syncthreads();
+      T r[8];
+      if (rrow) {
+        T ym[8], yp[8];
+        LdsRow<T>::load(LXrow - g.px, gx, ngx, ym);
+        LdsRow<T>::load(LXrow + g.px, gx, ngx, yp);
+        const T left = gx > 0 ? LdsRow<T>::left(LXrow, gx, ngx) : (T)0;
+        const T right = gx < ngx - 1 ? LdsRow<T>::right(LXrow, gx, ngx) : (T)0;
+  #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const T xl = q > 0 ? xc[q - 1] : left;
+          const T xr = q < 7 ? xc[q + 1] : right;
+          T a = cy * ym[q] + cx * xl + D * xc[q] + cx * xr + cy * yp[q];
+          a = cz * xm[q] + a + cz * xp[q];
+          const int i = 8 * gx + q;
+          r[q] = (rvalid && i >= 1 && i <= g.n0 - 1) ? fb[q] - a : (T)0;
         }
-#pragma unroll
-        for (int a = 0; a < 4; ++a) acc[a] = half * u[a];
-      } else {
-#pragma unroll
-        for (int a = 0; a < 4; ++a) acc[a] += u[a];
+        LdsRow<T>::store(LRrow, gx, ngx, r);
+      }
+      __syncthreads();
+      if (rrow) {
+        const T rl = gx > 0 ? LdsRow<T>::left(LRrow, gx, ngx) : (T)0;
+        T t[4];
+  #pragma unroll
+        for (int a = 0; a < 4; ++a) t[a] = half * (a > 0 ? r[2 * a - 1] : rl) + r[2 * a] + half * r[2 * a + 1];
+  #pragma unroll
+        for (int a = 0; a < 4; ++a) LTrow[a] = t[a];
+      }
+      __syncthreads();
+      if (crow) {
+        const int hp = g.px >> 1;
+        T u[4];
+  #pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = half * LTrow[a - hp] + LTrow[a] + half * LTrow[a + hp];
+        if (p & 1) {
+          // fine plane 2K+1 completes coarse plane K = (p-1)/2 and starts (p+1)/2
+          const int K = (p - 1) >> 1;
+          if (K >= Ks) {
+            T* out = fc + (long long)chain * gc.cs + ((long long)K * gc.py + J) * gc.px + 4 * gx;
+  #pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              const int I = 4 * gx + a;
+              if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a] + half * u[a];
+            }
+          }
+  #pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] = half * u[a];
+        } else {
+  #pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] += u[a];
+        }
+      }
+  #pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        xm[q] = xc[q];
+        xc[q] = xp[q];
+        xp[q] = xq[q];
+        fb[q] = fn[q];
       }
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      xm[q] = xc[q];
-      xc[q] = xp[q];
-      xp[q] = xq[q];
-      fb[q] = fn[q];
-    }
-  }
   };
 
   // (the host keeps ncol * (gc.n2 - 1) below 2^31; the run bounds are uniform)
