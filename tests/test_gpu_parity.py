@@ -246,12 +246,15 @@ def test_fused_sweep_matches_colour_phases(precision, tol, threads, blocks, bala
 
 
 @pytest.mark.parametrize("precision,n,K,threads", [("fp32", (128, 128, 128), 64, 512), ("fp64", (128, 128, 128), 1, 512),
-                                                   ("fp32", (48, 40, 36), 5, 64), ("fp64", (48, 40, 36), 3, 128)])
+                                                   ("fp32", (48, 40, 36), 5, 64), ("fp64", (48, 40, 36), 3, 128),
+                                                   ("fp32", (32, 24, 32), 300, 1024)])
 def test_balanced_sweep_split_gives_the_same_samples(precision, n, K, threads):
     """Whole sampling cycles (folded coarse correction, low-rank terms) with
     the level-0 sweep's (chain, tile, plane) space split evenly over one round
     of resident workgroups equal those of its z-segment split bit for bit,
-    among them the 128^3 posterior with 32 measurements and 64 FP32 chains."""
+    among them the 128^3 posterior with 32 measurements and 64 FP32 chains,
+    and 300 chains on a small lattice (more columns than resident workgroups:
+    a run crosses several columns)."""
     lat = mg.Lattice(*n)
     centres, y, var = mg.seeded_ball_measurements(lat, 32, 0.05, 1e-4, 3)
     op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.05, var)
