@@ -1482,11 +1482,14 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
         const uint32_t cj = (uint32_t)(j * ngx + gx);
         const uint32_t cpl = (uint32_t)(g.py * ngx);
         const uint32_t cA = ((k * cpl + cj) << 1) | (uint32_t)iparA, cB = ((kb * cpl + cj) << 1) | (uint32_t)iparB;
-        if (doA) normals4(cA, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiA);
-        if (doB) normals4(cB, (uint32_t)chain, nk.tag, sample, nk.k0, nk.k1, xiB);
+        uint32_t k0 = nk.k0, k1 = nk.k1;  // the key schedule per step (opaque_sgpr)
+        opaque_sgpr(k0);
+        opaque_sgpr(k1);
+        if (doA) normals4(cA, (uint32_t)chain, nk.tag, sample, k0, k1, xiA);
+        if (doB) normals4(cB, (uint32_t)chain, nk.tag, sample, k0, k1, xiB);
         if constexpr (CB > 1) {
-          if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiA1);
-          if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, nk.k0, nk.k1, xiB1);
+          if (doA) normals4(cA, (uint32_t)(chain + 1), nk.tag, sample, k0, k1, xiA1);
+          if (doB) normals4(cB, (uint32_t)(chain + 1), nk.tag, sample, k0, k1, xiB1);
         }
       }
       if (next) {
