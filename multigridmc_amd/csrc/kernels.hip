@@ -1429,11 +1429,11 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
 
     T xmm[8], xm[8], xc[8], xp[8], fm[8], fc[8];
     T xmm1[8], xm1[8], xc1[8], xp1[8];
-  #pragma unroll
+#pragma unroll
     for (int q = 0; q < 8; ++q) xmm[q] = xm[q] = xc[q] = xp[q] = fm[q] = fc[q] = xmm1[q] = xm1[q] = xc1[q] = xp1[q] = (T)0;
     // optional prolongate-add of the coarse correction while the planes are read
     ProlongStream<T> pro, pro1;
-  #define MGMC_RB3_INIT(XM, XC, PRO, CH, CBASE)                                                                    \
+#define MGMC_RB3_INIT(XM, XC, PRO, CH, CBASE)                                                                    \
     PRO.init(xcoarse && has_mem ? xcoarse + (long long)(CH) * gc.cs : nullptr, gc, gx, j, ks - 2 >= 0 ? ks - 2 : ks - 1, \
              alpha);                                                                                                 \
     if (has_mem && ks - 2 >= 0) {                                                                                    \
@@ -1448,7 +1448,7 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
     if constexpr (CB > 1) {
       MGMC_RB3_INIT(xm1, xc1, pro1, chain + 1, cbase1)
     }
-  #undef MGMC_RB3_INIT
+#undef MGMC_RB3_INIT
 
     // step k: xmm, xm, xc = planes k-2, k-1, k (xm after the first colour), xp
     // receives plane k+1; fm, fc = f on planes k-1, k
@@ -1461,7 +1461,7 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
       // (the coarse rows of plane k+1's correction are requested with it and
       // added after the noise, before the first colour reads plane k+1)
       const bool next = has_mem && k + 1 <= g.n2;
-  #define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
+#define MGMC_RB3_LOAD(XP, PRO, CBASE)                     \
     if (next) {                                             \
       load8<T>(x + CBASE + (long long)(k + 1) * pl, XP);    \
       PRO.advance(k + 1);                                   \
@@ -1472,7 +1472,7 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
       if constexpr (CB > 1) {
         MGMC_RB3_LOAD(xp1, pro1, cbase1)
       }
-  #undef MGMC_RB3_LOAD
+#undef MGMC_RB3_LOAD
       if (doA) load8<T>(f + fbase + (long long)k * pl, fc);
       // noise of both colours while the loads are in flight
       const int iparA = (first + j + k) & 1, iparB = (second + j + kb) & 1;
@@ -1501,7 +1501,7 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
         if (active) LdsRow<T>::store(A0 + csz + (k & 1) * asz + aoff, gx, ngx, xc1);
       }
       __syncthreads();
-  #define MGMC_RB3_UPDATE(XMM, XM, XC, XP, XIA, XIB, CBASE, AC)                                                      \
+#define MGMC_RB3_UPDATE(XMM, XM, XC, XP, XIA, XIB, CBASE, AC)                                                      \
     {                                                                                                                \
       const T* A = (AC) + (k & 1) * asz;                                                                             \
       const T* Bm = (AC) + 2 * asz + (kb & 1) * bsz; /* plane k-1 after the first colour (written last step) */     \
@@ -1551,8 +1551,8 @@ __device__ __forceinline__ void b_sweep_rb3(const Blk bk, const T* __restrict__ 
       if constexpr (CB > 1) {
         MGMC_RB3_UPDATE(xmm1, xm1, xc1, xp1, xiA1, xiB1, cbase1, A0 + csz)
       }
-  #undef MGMC_RB3_UPDATE
-  #pragma unroll
+#undef MGMC_RB3_UPDATE
+#pragma unroll
       for (int q = 0; q < 8; ++q) fm[q] = fc[q];
     }
   };
@@ -1928,7 +1928,7 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
     const long long fbase = (long long)chain * g.fcs + (long long)j * g.px + 8 * gx;
 
     T xm[8], xc[8], xp[8], xq[8], fb[8], fn[8];
-  #pragma unroll
+#pragma unroll
     for (int q = 0; q < 8; ++q) xm[q] = xc[q] = xp[q] = xq[q] = fb[q] = fn[q] = (T)0;
     const int p0 = 2 * Ks - 1, p1 = 2 * Ke - 1;  // p1 + 1 <= n2
     if (has_mem) {
@@ -1956,7 +1956,7 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
         LdsRow<T>::load(LXrow + g.px, gx, ngx, yp);
         const T left = gx > 0 ? LdsRow<T>::left(LXrow, gx, ngx) : (T)0;
         const T right = gx < ngx - 1 ? LdsRow<T>::right(LXrow, gx, ngx) : (T)0;
-  #pragma unroll
+#pragma unroll
         for (int q = 0; q < 8; ++q) {
           const T xl = q > 0 ? xc[q - 1] : left;
           const T xr = q < 7 ? xc[q + 1] : right;
@@ -1971,36 +1971,36 @@ __device__ __forceinline__ void b_residual_restrict_star3(const Blk bk, const T*
       if (rrow) {
         const T rl = gx > 0 ? LdsRow<T>::left(LRrow, gx, ngx) : (T)0;
         T t[4];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 4; ++a) t[a] = half * (a > 0 ? r[2 * a - 1] : rl) + r[2 * a] + half * r[2 * a + 1];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 4; ++a) LTrow[a] = t[a];
       }
       __syncthreads();
       if (crow) {
         const int hp = g.px >> 1;
         T u[4];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 4; ++a) u[a] = half * LTrow[a - hp] + LTrow[a] + half * LTrow[a + hp];
         if (p & 1) {
           // fine plane 2K+1 completes coarse plane K = (p-1)/2 and starts (p+1)/2
           const int K = (p - 1) >> 1;
           if (K >= Ks) {
             T* out = fc + (long long)chain * gc.cs + ((long long)K * gc.py + J) * gc.px + 4 * gx;
-  #pragma unroll
+#pragma unroll
             for (int a = 0; a < 4; ++a) {
               const int I = 4 * gx + a;
               if (I >= 1 && I <= gc.n0 - 1) out[a] = acc[a] + half * u[a];
             }
           }
-  #pragma unroll
+#pragma unroll
           for (int a = 0; a < 4; ++a) acc[a] = half * u[a];
         } else {
-  #pragma unroll
+#pragma unroll
           for (int a = 0; a < 4; ++a) acc[a] += u[a];
         }
       }
-  #pragma unroll
+#pragma unroll
       for (int q = 0; q < 8; ++q) {
         xm[q] = xc[q];
         xc[q] = xp[q];
