@@ -36,3 +36,29 @@ def test_autotune_keeps_state_and_sample_path(precision):
     plain.run(3, record=False)
     tuned.run(3, record=False)
     assert np.array_equal(tuned.get_state(), plain.get_state())
+
+
+def test_tuning_keys_read_back_and_reject_bad_values():
+    """The launch-shape keys read back what was set (defaults: balanced split
+    on, carried residual term on, 512 entries of B per dot workgroup at 8
+    chains and more, 128 below); out-of-range values and unknown keys raise
+    MGMC_ERR_ARG and leave the setting as it was."""
+    lat = mg.Lattice(16, 16, 16)
+    centres, y, var = mg.seeded_ball_measurements(lat, 3, 0.1, 1e-3, 5)
+    op = mg.MeasuredOperator(mg.ShiftedLaplaceFDOperator(lat, 10.0), centres, 0.1, var)
+    p = mg.MultigridParameters(nlevel=3)
+    for K, split in ((8, 512), (2, 128)):
+        s = mg.MultigridMCSampler(op, p, nchains=K, precision="fp32", seed=1)
+        assert s.get_tuning("rb_balanced") == 1
+        assert s.get_tuning("lr_carry") == 1
+        assert s.get_tuning("lr_split") == split
+        s.set_tuning("lr_split", 64)
+        assert s.get_tuning("lr_split") == 64
+        for key, bad in (("lr_split", 63), ("rb_min_planes", 0), ("rb_threads", 2048), ("lr_dot_chains", 3)):
+            before = s.get_tuning(key)
+            with pytest.raises(mg.MGMCError) as e:
+                s.set_tuning(key, bad)
+            assert "ERR_ARG" in str(e.value)
+            assert s.get_tuning(key) == before
+        with pytest.raises(mg.MGMCError):
+            s.set_tuning("no_such_key", 1)
